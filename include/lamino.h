@@ -1,0 +1,191 @@
+/*
+ * lamino.h -- C ABI of the MI355X (gfx950) vector-laminography library.
+ *
+ * Each entry point replaces one function of the reference (lbl-camera/lamino)
+ * on its reconstruction hot path; the comment on each cites the interface it
+ * stands in for.  Conventions:
+ *   - volumes are row-major float32 [n1][n2][n3] (dims_t, include/dtypes.h:31-84);
+ *     dims[] = {n1, n2, n3};
+ *   - projections are float32 [nproj][nrows][ncols];
+ *   - complex arrays are interleaved float32 pairs (std::complex<float>);
+ *   - unless the name says "host", array arguments are DEVICE pointers and
+ *     the work is ordered on the context's stream;
+ *   - every call returns LAM_OK or an error code; lam_last_error() gives the
+ *     message, and the C++ layer (include/tomocam/) rethrows the exception
+ *     the reference would throw.
+ */
+#ifndef LAMINO_H
+#define LAMINO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LAM_OK = 0,
+    LAM_ERR_ARG = 1,      /* std::invalid_argument */
+    LAM_ERR_HIP = 2,      /* HIP runtime failure */
+    LAM_ERR_NOMEM = 3,    /* std::bad_alloc (include/fft.h:69) */
+    LAM_ERR_NUFFT = 4,    /* "Error in finufft_*" (include/nufft.h:48,63) */
+    LAM_ERR_SHAPE = 5,    /* split-Bregman x0/yT mismatch (src/bregman.cpp:44-48) */
+    LAM_ERR_TYPE = 6,     /* NUFFT type not in {1,2} (include/finufft_plan_cache.h:63-64) */
+    LAM_ERR_DIVZERO = 7,  /* "Division by zero" (include/array.h:163-165) */
+    LAM_ERR_STATE = 8,    /* job / handle used in the wrong state */
+    LAM_ERR_BUDGET = 9    /* request exceeds the device memory budget */
+};
+typedef int lam_status;
+
+typedef struct lam_ctx lam_ctx;    /* device, stream, plan cache, scratch */
+typedef struct lam_grid lam_grid;  /* polar grid: per-projection plane geometry */
+typedef struct lam_toep lam_toep;  /* Toeplitz normal operator (sum over grids) */
+
+/* Message of the last failed call on this thread. */
+const char *lam_last_error(void);
+
+/* ------------------------------------------------------------------ context */
+lam_status lam_ctx_create(int device, lam_ctx **out);
+lam_status lam_ctx_destroy(lam_ctx *ctx);
+/* Use an external hipStream_t (NULL restores the library's own stream). */
+lam_status lam_ctx_set_stream(lam_ctx *ctx, void *hip_stream);
+void *lam_ctx_stream(lam_ctx *ctx);
+lam_status lam_ctx_sync(lam_ctx *ctx);
+
+/* Device memory helpers (stream-ordered copies on the context's stream). */
+lam_status lam_malloc(lam_ctx *ctx, size_t bytes, void **ptr);
+lam_status lam_free(lam_ctx *ctx, void *ptr);
+lam_status lam_upload(lam_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+lam_status lam_download(lam_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+lam_status lam_memset(lam_ctx *ctx, void *dst_dev, int value, size_t bytes);
+
+/* -------------------------------------------------------------------- grids */
+/* PolarGrid<T>(angles, nrows, ncols, gamma), include/polar_grid.h:44-75.
+ * Points are generated analytically on the device; nothing of size P*R*C is stored. */
+lam_status lam_grid_polar(lam_ctx *ctx, const double *theta_host, int64_t nproj, int64_t nrows, int64_t ncols,
+                          double gamma, lam_grid **out);
+/* Per-projection generalisation (alpha_p, gamma_p): the reference ctor is alpha = theta,
+ * gamma_p = gamma.  Used for the laminography scan (alpha = tilt, gamma = offset + rotation). */
+lam_status lam_grid_polar_pp(lam_ctx *ctx, const double *alpha_host, const double *gamma_host, int64_t nproj,
+                             int64_t nrows, int64_t ncols, lam_grid **out);
+lam_status lam_grid_destroy(lam_grid *grid);
+lam_status lam_grid_shape(const lam_grid *grid, int64_t *nproj, int64_t *nrows, int64_t *ncols);
+/* Materialise PolarGrid's public x, y, z fields (host float arrays of P*R*C),
+ * computed as the reference computes them (include/polar_grid.h:61-78). */
+lam_status lam_grid_coords_host(const lam_grid *grid, float *x, float *y, float *z);
+
+/* --------------------------------------------------------------------- NUFFT */
+/* nufft::nufft3d1 (include/nufft.h:37-50): f_k = sum_p c_p exp(+i k.x_p);
+ * c: complex [P][R][C], f: complex volume dims.  eps <= 0 selects the default (1e-5). */
+lam_status lam_nufft3d1(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const void *c, void *f);
+/* nufft::nufft3d2 (include/nufft.h:52-65): c_p = sum_k f_k exp(-i k.x_p). */
+lam_status lam_nufft3d2(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, void *c, const void *f);
+
+/* --------------------------------------------------------------- projectors */
+/* forward (include/projection.h:52-54, src/projection.cpp:41-77) */
+lam_status lam_forward(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *m0,
+                       const float *m1, const float *m2, float *proj);
+/* adjoint (include/projection.h:64-66, src/projection.cpp:89-126) */
+lam_status lam_adjoint(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *proj,
+                       float *m0, float *m1, float *m2);
+/* adjoint with accumulation into m (the yT += adjoint(...) loop of src/mbir_*.cpp) */
+lam_status lam_adjoint_accumulate(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps,
+                                  const float *proj, float *m0, float *m1, float *m2);
+/* sysmat (include/tomocam.h:50-52, src/gradient.cpp:33-89), NUFFT form */
+lam_status lam_sysmat(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *const x[3],
+                      float *const y[3]);
+
+/* ---------------------------------------------------- Toeplitz normal operator */
+/* Builds the exact (closed-form) Toeplitz kernels of sum_g sysmat_g for the
+ * given grids.  keep_per_grid != 0 additionally keeps each grid's kernels for
+ * lam_toep_quad (the per-dataset residual of src/gradient.cpp:115-127). */
+lam_status lam_toep_create(lam_ctx *ctx, lam_grid *const *grids, int ngrids, const int64_t dims[3],
+                           int keep_per_grid, lam_toep **out);
+lam_status lam_toep_destroy(lam_toep *op);
+/* y = sum_g sysmat_g(x) (+ mu * laplacian(x) per component when mu != 0) */
+lam_status lam_toep_apply(lam_toep *op, const float *const x[3], float *const y[3], float mu);
+/* q[g*3+i] = x_i . sysmat_g(x)_i for every grid g (host output, FP64). */
+lam_status lam_toep_quad(lam_toep *op, const float *const x[3], double *q_host);
+
+/* ---------------------------------------------------------------- spectral */
+/* fft::fft2 (sign -1) / fft::ifft2 (sign +1), include/fft.h:34-114: batched,
+ * unnormalised 2-D complex transform over (n2, n3) for each of n1 slices.
+ * shifted != 0 wraps it as ifftshift2(fft(fftshift2(x))) (src/projection.cpp:73-75). */
+lam_status lam_fft2(lam_ctx *ctx, const int64_t dims[3], int sign, int shifted, const void *in, void *out);
+
+/* ------------------------------------------------------ stencils, elementwise */
+/* opt::RampPreconditioner<T>(dims).apply(in) (include/precond.h:33-87) */
+lam_status lam_precond(lam_ctx *ctx, const int64_t dims[3], const float *in, float *out);
+/* opt::grad_u (src/finitediff.cpp:7-47) */
+lam_status lam_grad_u(lam_ctx *ctx, const int64_t dims[3], const float *u, float *g0, float *g1, float *g2);
+/* opt::divergence (src/finitediff.cpp:53-71) */
+lam_status lam_divergence(lam_ctx *ctx, const int64_t dims[3], const float *v0, const float *v1, const float *v2,
+                          float *out);
+/* opt::laplacian = divergence(grad_u(u)) (include/bregman.h:33-36), fused */
+lam_status lam_laplacian(lam_ctx *ctx, const int64_t dims[3], const float *u, float *out);
+/* opt::qggmrf (src/qggmrf.cpp:59-95): g += prior gradient, in place */
+lam_status lam_qggmrf(lam_ctx *ctx, const int64_t dims[3], float *g, const float *x, float sigma, float p);
+/* array::dot / norm2 / max (include/array_ops.h:63-93), FP64 accumulation, fixed order */
+lam_status lam_dot(lam_ctx *ctx, int64_t n, const float *a, const float *b, double *out_host);
+lam_status lam_max(lam_ctx *ctx, int64_t n, const float *a, float *out_host);
+
+/* ------------------------------------------------------------------- solvers */
+/* Operator callback: y = A(x) on device arrays, stream-ordered on the context stream. */
+typedef lam_status (*lam_op_fn)(void *user, const float *const x[3], float *const y[3]);
+/* Loss callback (host scalar). */
+typedef lam_status (*lam_loss_fn)(void *user, const float *const x[3], double *loss);
+
+/* opt::cgsolver (include/optimize.h:89-91, src/conjgrad.cpp:36-117); FP64 scalars. */
+lam_status lam_cgsolver(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void *user, const float *const b[3],
+                        const float *const x0[3], int64_t max_iters, double tol, float *const x[3],
+                        int64_t *iters_done);
+/* opt::split_bregman (include/optimize.h:74-76, src/bregman.cpp:38-134) */
+lam_status lam_split_bregman(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void *user, const float *const yT[3],
+                             const float *const x0[3], double lambda, double mu, int64_t outer_max,
+                             int64_t inner_max, double tol, double xtol, float *const x[3]);
+/* opt::nagopt, vector form (include/optimize.h:107-112, src/nagopt.cpp:115-209);
+ * x is updated in place as in the reference. */
+lam_status lam_nagopt(lam_ctx *ctx, const int64_t dims[3], lam_op_fn grad, lam_loss_fn loss, void *user,
+                      float *const x[3], int64_t max_iters, double lipschitz, double tol, double xtol,
+                      int64_t max_inner);
+/* opt::lipschitz (include/optimize.h:140-142, src/lipschitz.cpp:30-53): FP64 power iteration. */
+lam_status lam_lipschitz(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void *user, int64_t max_iters,
+                         double tol, uint64_t seed, double *L);
+
+/* ---------------------------------------------------------- reconstruction jobs */
+typedef struct {
+    int64_t nproj, nrows, ncols;  /* projection stack dims */
+    const float *proj;            /* HOST [nproj][nrows][ncols] */
+    const double *alpha;          /* HOST per projection (the reference's angles) */
+    const double *gamma;          /* HOST per projection, or NULL to use gamma_const */
+    double gamma_const;
+} lam_dataset;
+
+typedef struct {
+    int method;             /* 1 MBIR1 (CG), 2 MBIR2 (split-Bregman TV), 3 MBIR3 (NAG + qGGMRF) */
+    int64_t max_iters;      /* ReconParams::maxIters */
+    int64_t inner_iters;    /* ReconParams::innerIters */
+    double lambda, mu, sigma, p, tol, xtol;
+    double pad_factor;      /* ReconParams::PAD_FACTOR (1.4142) */
+    double eps;             /* NUFFT tolerance for the backprojection (<= 0: default) */
+} lam_recon_params;
+
+typedef struct {
+    double setup_s;         /* normalisation, padding, backprojection, Toeplitz build */
+    double solve_s;         /* solver loop */
+    int64_t normal_ops;     /* normal-operator applications */
+    int64_t cg_iters;       /* inner CG iterations executed */
+    int64_t outer_iters;    /* outer iterations executed */
+} lam_recon_stats;
+
+/* MBIR1/2/3 (include/tomocam.h:101-154; src/mbir_cg.cpp, src/mbir_sb.cpp, src/mbir3.cpp).
+ * out3_host receives the 3 cropped components [3][n1][n2][n3]. */
+lam_status lam_mbir(lam_ctx *ctx, const lam_dataset *datasets, int ndatasets, const int64_t recon_dims[3],
+                    const lam_recon_params *params, float *out3_host, lam_recon_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAMINO_H */

@@ -1,0 +1,57 @@
+// Batched 1-D FFT passes over one axis of a strided array, in LDS.
+//
+// A pass transforms a set of lines indexed by (o1, o2, i); element n of line
+// (o1, o2, i) lives at  o1*s1 + o2*s2 + i*si + n*sn  (separate strides for the
+// input and the output, in elements of their own type).  One workgroup owns
+// L lines: consecutive i when ci > 1 (so that the loads of a row of the tile
+// are contiguous), otherwise consecutive o2.  The transform is a mixed-radix
+// Stockham autosort (radices 2,3,4,5,7,11,13) or, for lengths with a larger
+// prime factor, Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length;
+// both run entirely in LDS between one load and one store.
+//
+// Fused at the load:  rotation (fftshift), zero padding beyond nv, real or
+// real-pair (re from in, im from in2) input.  Fused at the store: rotation
+// (ifftshift), truncation to ns, scaling, real or real-pair output.
+#pragma once
+#include "common.h"
+
+namespace lam {
+
+enum { LD_COMPLEX = 0, LD_REAL = 1, LD_REAL_PAIR = 2 };
+enum { ST_COMPLEX = 0, ST_REAL = 1, ST_REAL_PAIR = 2, ST_IMAG = 3 };
+
+struct FftPlan {
+    int n = 0, sign = -1, m = 0;
+    bool blue = false;
+    int nrad = 0;
+    int rad[24] = {};
+    DevBuf roots;  // m entries exp(-2 pi i k / m)
+    DevBuf chirp;  // n entries exp(sign pi i k^2 / n)        (Bluestein)
+    DevBuf bhat;   // m entries FFT_m(conj chirp, wrapped) / m (Bluestein)
+};
+
+struct AxisPass {
+    const void *in = nullptr, *in2 = nullptr;
+    void *out = nullptr, *out2 = nullptr;
+    int ld_mode = LD_COMPLEX, st_mode = ST_COMPLEX;
+    long long is1 = 0, is2 = 0, isi = 0, isn = 1;
+    long long os1 = 0, os2 = 0, osi = 0, osn = 1;
+    int c1 = 1, c2 = 1, ci = 1;
+    int nv = -1, ns = -1;      // -1: full length
+    int rot_in = 0, rot_out = 0;
+    float scale = 1.0f;
+};
+
+void run_axis(Ctx &cx, const FftPlan &plan, AxisPass p);
+
+// Convenience: the reference's centred 2-D detector transform
+//   ifftshift2(fft2|ifft2(fftshift2(in)))  (include/fft.h:34-114,
+//   include/fftutils.h:84-168, as used in src/projection.cpp:73-75, 94-99)
+// over P planes of R x C.  in_real: float input; out_real: store Re()*scale.
+void det_fft2(Ctx &cx, const void *in, bool in_real, void *out, bool out_real, int P, int R, int C, int sign,
+              float scale, float2 *tmp);
+
+// Full in-place 3-D complex FFT of [A][B][Cn][nc] (nc interleaved components).
+void fft3d(Ctx &cx, float2 *data, int A, int B, int Cn, int nc, int sign);
+
+}  // namespace lam

@@ -1,0 +1,400 @@
+// LDS-resident mixed-radix Stockham / Bluestein FFT passes for gfx950.
+#include <cmath>
+#include <complex>
+
+#include "fft.h"
+
+namespace lam {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsComplex = 4096;  // per buffer: L * m <= 4096 complex -> 2 buffers = 64 KiB
+
+struct KParams {
+    AxisPass a;
+    int n, m, blue, nrad, L, sign;
+    int rad[24];
+    const float2 *roots, *chirp, *bhat;
+};
+
+template <int INNER>
+__device__ __forceinline__ int lidx(int l, int n, int L, int M) {
+    return INNER ? n * L + l : l * M + n;
+}
+
+__device__ __forceinline__ float2 twiddle(const float2 *roots, int idx, int sign) {
+    float2 w = roots[idx];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+// v'[s] = sum_t v[t] exp(sign 2 pi i t s / R)
+template <int R>
+__device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, int mstep) {
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const float s3 = 0.86602540378443864676f * (float)sign;
+        float2 a = v[0], b = v[1], c = v[2];
+        float2 t1 = cadd(b, c);
+        float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+        float2 d = csub(b, c);
+        float2 t3 = make_float2(-s3 * d.y, s3 * d.x);  // i * s3 * (b - c)
+        v[0] = cadd(a, t1);
+        v[1] = cadd(t2, t3);
+        v[2] = csub(t2, t3);
+    } else if constexpr (R == 4) {
+        float2 a = v[0], b = v[1], c = v[2], d = v[3];
+        float2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+        float2 is3 = make_float2(-(float)sign * s3.y, (float)sign * s3.x);  // sign * i * (b - d)
+        v[0] = cadd(s0, s2);
+        v[1] = cadd(s1, is3);
+        v[2] = csub(s0, s2);
+        v[3] = csub(s1, is3);
+    } else if constexpr (R == 5) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f * (float)sign, s2 = 0.58778525229247312917f * (float)sign;
+        float2 a = v[0];
+        float2 be = cadd(v[1], v[4]), bm = csub(v[1], v[4]);
+        float2 cd = cadd(v[2], v[3]), cm = csub(v[2], v[3]);
+        float2 r1 = make_float2(a.x + c1 * be.x + c2 * cd.x, a.y + c1 * be.y + c2 * cd.y);
+        float2 r2 = make_float2(a.x + c2 * be.x + c1 * cd.x, a.y + c2 * be.y + c1 * cd.y);
+        float2 i1 = make_float2(s1 * bm.x + s2 * cm.x, s1 * bm.y + s2 * cm.y);
+        float2 i2 = make_float2(s2 * bm.x - s1 * cm.x, s2 * bm.y - s1 * cm.y);
+        v[0] = make_float2(a.x + be.x + cd.x, a.y + be.y + cd.y);
+        v[1] = make_float2(r1.x - i1.y, r1.y + i1.x);
+        v[4] = make_float2(r1.x + i1.y, r1.y - i1.x);
+        v[2] = make_float2(r2.x - i2.y, r2.y + i2.x);
+        v[3] = make_float2(r2.x + i2.y, r2.y - i2.x);
+    } else {
+        // generic small prime: W_R^x = roots[(x mod R) * mstep] (mstep = m / R)
+        float2 o[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int t = 1; t < R; ++t) acc = cadd(acc, cmul(v[t], twiddle(roots, ((t * s) % R) * mstep, sign)));
+            o[s] = acc;
+        }
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[s] = o[s];
+    }
+}
+
+template <int R, int INNER>
+__device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__restrict__ dst, int L, int M,
+                                      int Ns, int sign, const float2 *roots) {
+    const int nbf = M / R;
+    const int tstep = M / (Ns * R);
+    for (int e = threadIdx.x; e < L * nbf; e += blockDim.x) {
+        int l, j;
+        if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
+        const int k = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = src[lidx<INNER>(l, j + t * nbf, L, M)];
+        if (Ns > 1) {
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], twiddle(roots, t * k * tstep, sign));
+        }
+        dft<R>(v, sign, roots, M / R);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) dst[lidx<INNER>(l, base + t * Ns, L, M)] = v[t];
+    }
+}
+
+template <int INNER>
+__device__ void stages(float2 *&src, float2 *&dst, const KParams &P, int sign) {
+    int Ns = 1;
+    for (int s = 0; s < P.nrad; ++s) {
+        const int r = P.rad[s];
+        switch (r) {
+            case 2: stage<2, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 3: stage<3, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 4: stage<4, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 5: stage<5, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 7: stage<7, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 11: stage<11, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            default: stage<13, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+        }
+        __syncthreads();
+        float2 *t = src; src = dst; dst = t;
+        Ns *= r;
+    }
+}
+
+template <int INNER>
+__global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
+    extern __shared__ float2 lds[];
+    const int L = P.L, M = P.m, N = P.n;
+    float2 *bufA = lds, *bufB = lds + (size_t)L * M;
+    const AxisPass &a = P.a;
+    const long long o1 = blockIdx.z;
+    const int o2base = INNER ? (int)blockIdx.y : (int)blockIdx.x * L;
+    const int ibase = INNER ? (int)blockIdx.x * L : (int)blockIdx.y;
+    const int nv = a.nv < 0 ? N : a.nv;
+    const int ns = a.ns < 0 ? N : a.ns;
+
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        int l, n;
+        if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
+        const int o2 = INNER ? o2base : o2base + l;
+        const int i = INNER ? ibase + l : ibase;
+        int mm = n + a.rot_in;
+        if (mm >= N) mm -= N;
+        float2 v = make_float2(0.f, 0.f);
+        if (o2 < a.c2 && i < a.ci && mm < nv) {
+            const long long off = o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi + (long long)mm * a.isn;
+            if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
+            else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
+            else {
+                v.x = static_cast<const float *>(a.in)[off];
+                v.y = static_cast<const float *>(a.in2)[off];
+            }
+        }
+        if (P.blue) v = cmul(v, P.chirp[n]);
+        bufA[lidx<INNER>(l, n, L, M)] = v;
+    }
+    if (P.blue) {
+        const int pad = M - N;
+        for (int e = threadIdx.x; e < L * pad; e += blockDim.x) {
+            int l, n;
+            if (INNER) { l = e % L; n = N + e / L; } else { n = N + e % pad; l = e / pad; }
+            bufA[lidx<INNER>(l, n, L, M)] = make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+
+    float2 *src = bufA, *dst = bufB;
+    if (!P.blue) {
+        stages<INNER>(src, dst, P, P.sign);
+    } else {
+        stages<INNER>(src, dst, P, -1);
+        for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
+            int l, n;
+            if (INNER) { l = e % L; n = e / L; } else { n = e % M; l = e / M; }
+            const int id = lidx<INNER>(l, n, L, M);
+            src[id] = cmul(src[id], P.bhat[n]);
+        }
+        __syncthreads();
+        stages<INNER>(src, dst, P, +1);
+    }
+
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        int l, n;
+        if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
+        const int o2 = INNER ? o2base : o2base + l;
+        const int i = INNER ? ibase + l : ibase;
+        int mm = n + a.rot_out;
+        if (mm >= N) mm -= N;
+        if (o2 < a.c2 && i < a.ci && mm < ns) {
+            float2 v = src[lidx<INNER>(l, n, L, M)];
+            if (P.blue) v = cmul(v, P.chirp[n]);
+            v.x *= a.scale;
+            v.y *= a.scale;
+            const long long off = o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi + (long long)mm * a.osn;
+            if (a.st_mode == ST_COMPLEX) static_cast<float2 *>(a.out)[off] = v;
+            else if (a.st_mode == ST_REAL) static_cast<float *>(a.out)[off] = v.x;
+            else if (a.st_mode == ST_IMAG) static_cast<float *>(a.out)[off] = v.y;
+            else {
+                static_cast<float *>(a.out)[off] = v.x;
+                static_cast<float *>(a.out2)[off] = v.y;
+            }
+        }
+    }
+}
+
+// radix list for a length, largest-first power-of-four grouping; false if a
+// prime factor above 13 remains
+bool factor_len(int n, int *rad, int &nrad) {
+    nrad = 0;
+    int r = n;
+    int twos = 0;
+    while (r % 2 == 0) { r /= 2; ++twos; }
+    for (int k = 0; k < twos / 2; ++k) rad[nrad++] = 4;
+    if (twos % 2) rad[nrad++] = 2;
+    for (int p : {3, 5, 7, 11, 13}) {
+        while (r % p == 0) { rad[nrad++] = p; r /= p; }
+    }
+    return r == 1;
+}
+
+bool smooth235(int n) {
+    for (int p : {2, 3, 5}) while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+}  // namespace
+
+FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
+    auto key = std::make_pair(n, sign);
+    auto it = cx.fft_plans.find(key);
+    if (it != cx.fft_plans.end()) return *it->second;
+    LAM_REQUIRE(n >= 1, LAM_ERR_ARG, "FFT length must be positive");
+    auto plan = std::make_unique<FftPlan>();
+    plan->n = n;
+    plan->sign = sign;
+    int rad[24], nrad;
+    if (n == 1) {
+        plan->m = 1;
+        plan->nrad = 0;
+    } else if (factor_len(n, rad, nrad)) {
+        plan->m = n;
+        plan->nrad = nrad;
+        for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
+    } else {
+        plan->blue = true;
+        int m = 2 * n - 1;
+        while (!smooth235(m)) ++m;
+        plan->m = m;
+        factor_len(m, rad, nrad);
+        plan->nrad = nrad;
+        for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
+    }
+    const int m = plan->m;
+    LAM_REQUIRE(m <= kLdsComplex, LAM_ERR_ARG, "FFT length " + std::to_string(n) + " exceeds the LDS transform limit");
+    const long double twopi = 6.283185307179586476925286766559L;
+    std::vector<float2> roots(m);
+    for (int k = 0; k < m; ++k) {
+        long double ang = -twopi * (long double)k / (long double)m;
+        roots[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    plan->roots.alloc(sizeof(float2) * m);
+    LAM_HIP(hipMemcpy(plan->roots.ptr, roots.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+    if (plan->blue) {
+        std::vector<std::complex<double>> ch(n), b(m, 0.0), bh(m);
+        std::vector<float2> chf(n), bhf(m);
+        for (int k = 0; k < n; ++k) {
+            unsigned long long q = (unsigned long long)k * (unsigned long long)k % (2ull * (unsigned long long)n);
+            long double ang = (long double)sign * (twopi / 2) * (long double)q / (long double)n;
+            ch[k] = std::complex<double>((double)std::cos(ang), (double)std::sin(ang));
+            chf[k] = make_float2((float)ch[k].real(), (float)ch[k].imag());
+        }
+        for (int k = 0; k < n; ++k) {
+            b[k] = std::conj(ch[k]);
+            if (k) b[m - k] = std::conj(ch[k]);
+        }
+        // direct DFT in double with an exact root table (m <= 4096)
+        std::vector<std::complex<double>> rt(m);
+        for (int k = 0; k < m; ++k) {
+            long double ang = -twopi * (long double)k / (long double)m;
+            rt[k] = std::complex<double>((double)std::cos(ang), (double)std::sin(ang));
+        }
+        for (int f = 0; f < m; ++f) {
+            std::complex<double> acc = 0.0;
+            long long idx = 0;
+            for (int k = 0; k < m; ++k) {
+                if (b[k] != 0.0) acc += b[k] * rt[idx];
+                idx += f;
+                if (idx >= m) idx -= m;
+            }
+            bh[f] = acc / (double)m;
+            bhf[f] = make_float2((float)bh[f].real(), (float)bh[f].imag());
+        }
+        plan->chirp.alloc(sizeof(float2) * n);
+        plan->bhat.alloc(sizeof(float2) * m);
+        LAM_HIP(hipMemcpy(plan->chirp.ptr, chf.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+        LAM_HIP(hipMemcpy(plan->bhat.ptr, bhf.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+    }
+    FftPlan &ref = *plan;
+    cx.fft_plans.emplace(key, std::move(plan));
+    return ref;
+}
+
+void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
+    if (a.c1 <= 0 || a.c2 <= 0 || a.ci <= 0) return;
+    KParams P;
+    P.a = a;
+    P.n = plan.n;
+    P.m = plan.m;
+    P.blue = plan.blue ? 1 : 0;
+    P.nrad = plan.nrad;
+    for (int i = 0; i < 24; ++i) P.rad[i] = plan.rad[i];
+    P.sign = plan.sign;
+    P.roots = plan.roots.as<float2>();
+    P.chirp = plan.chirp.as<float2>();
+    P.bhat = plan.bhat.as<float2>();
+    const bool inner = a.ci > 1;
+    const int cap = std::max(1, kLdsComplex / plan.m);
+    int L;
+    dim3 grid;
+    if (inner) {
+        L = std::min({a.ci, cap, 16});
+        grid = dim3((unsigned)((a.ci + L - 1) / L), (unsigned)a.c2, (unsigned)a.c1);
+    } else {
+        L = std::min({a.c2, cap, 32});
+        grid = dim3((unsigned)((a.c2 + L - 1) / L), (unsigned)a.ci, (unsigned)a.c1);
+    }
+    LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
+    P.L = L;
+    size_t lds = 2 * (size_t)L * plan.m * sizeof(float2);
+    if (inner)
+        hipLaunchKernelGGL(k_fft_axis<1>, grid, dim3(kThreads), lds, cx.stream, P);
+    else
+        hipLaunchKernelGGL(k_fft_axis<0>, grid, dim3(kThreads), lds, cx.stream, P);
+    check_launch();
+}
+
+void det_fft2(Ctx &cx, const void *in, bool in_real, void *out, bool out_real, int P, int R, int C, int sign,
+              float scale, float2 *tmp) {
+    // pass 1: along columns (contiguous), lines (p, r)
+    {
+        AxisPass a;
+        a.in = in;
+        a.ld_mode = in_real ? LD_REAL : LD_COMPLEX;
+        a.is1 = (long long)R * C; a.is2 = C; a.isi = 0; a.isn = 1;
+        a.out = tmp;
+        a.st_mode = ST_COMPLEX;
+        a.os1 = (long long)R * C; a.os2 = C; a.osi = 0; a.osn = 1;
+        a.c1 = P; a.c2 = R; a.ci = 1;
+        a.rot_in = C / 2;
+        a.rot_out = C / 2;
+        run_axis(cx, get_fft_plan(cx, C, sign), a);
+    }
+    // pass 2: along rows, lines (p, c), tiled over c
+    {
+        AxisPass a;
+        a.in = tmp;
+        a.ld_mode = LD_COMPLEX;
+        a.is1 = (long long)R * C; a.is2 = 0; a.isi = 1; a.isn = C;
+        a.out = out;
+        a.st_mode = out_real ? ST_REAL : ST_COMPLEX;
+        a.os1 = (long long)R * C; a.os2 = 0; a.osi = 1; a.osn = C;
+        a.c1 = P; a.c2 = 1; a.ci = C;
+        a.rot_in = R / 2;
+        a.rot_out = R / 2;
+        a.scale = scale;
+        run_axis(cx, get_fft_plan(cx, R, sign), a);
+    }
+}
+
+void fft3d(Ctx &cx, float2 *data, int A, int B, int Cn, int nc, int sign) {
+    const long long sC = nc, sB = (long long)Cn * nc, sA = (long long)B * Cn * nc;
+    {   // axis C (x): lines (a, b) x components
+        AxisPass p;
+        p.in = data; p.out = data;
+        p.is1 = p.os1 = sA; p.is2 = p.os2 = sB; p.isi = p.osi = 1; p.isn = p.osn = sC;
+        p.c1 = A; p.c2 = B; p.ci = nc;
+        run_axis(cx, get_fft_plan(cx, Cn, sign), p);
+    }
+    {   // axis B (y): lines (a, c*nc)
+        AxisPass p;
+        p.in = data; p.out = data;
+        p.is1 = p.os1 = sA; p.is2 = p.os2 = 0; p.isi = p.osi = 1; p.isn = p.osn = sB;
+        p.c1 = A; p.c2 = 1; p.ci = Cn * nc;
+        run_axis(cx, get_fft_plan(cx, B, sign), p);
+    }
+    {   // axis A (z): lines (b, c*nc)
+        AxisPass p;
+        p.in = data; p.out = data;
+        p.is1 = p.os1 = 0; p.is2 = p.os2 = sB; p.isi = p.osi = 1; p.isn = p.osn = sA;
+        p.c1 = 1; p.c2 = B; p.ci = Cn * nc;
+        run_axis(cx, get_fft_plan(cx, A, sign), p);
+    }
+}
+
+}  // namespace lam

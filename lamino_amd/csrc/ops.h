@@ -1,0 +1,48 @@
+// Fused elementwise, stencil and reduction kernels on [n1][n2][n3] volumes.
+#pragma once
+#include "common.h"
+
+namespace lam {
+
+struct Vol3 {
+    float *c[3];
+};
+struct CVol3 {
+    const float *c[3];
+};
+
+// ---- reductions (FP64, fixed launch shape => deterministic order)
+// Term kinds: sum a*b, sum (a-b)^2, sum a^2
+enum { RED_DOT = 0, RED_DIFF2 = 1, RED_SQ = 2 };
+struct RedTerm {
+    int kind;
+    const float *a, *b;
+};
+constexpr int kMaxRedTerms = 12;
+// Computes sums of up to 12 terms over n elements; results to host (blocking).
+void reduce_terms(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_host);
+// Same, results left in a device double array (no sync).
+void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev);
+float max_value(Ctx &cx, long long n, const float *a);
+
+// ---- stencils (src/finitediff.cpp:7-71, include/bregman.h:33-36)
+void grad_u(Ctx &cx, const long long d[3], const float *u, float *g0, float *g1, float *g2);
+void divergence(Ctx &cx, const long long d[3], const float *v0, const float *v1, const float *v2, float *out);
+// out = laplacian(u)  or  y_c += mu * laplacian(x_c) for the 3 components
+void laplacian(Ctx &cx, const long long d[3], const float *u, float *out);
+void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu);
+
+// ---- qGGMRF prior gradient (src/qggmrf.cpp:59-95): g_c += prior(x_c), c < ncomp
+void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma, float p);
+
+// ---- ramp preconditioner (include/precond.h:33-87), out = M in, on 1 or 2 volumes at once
+void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1);
+
+// ---- elementwise helpers
+// y = a*x + b*y over n floats (3 comps, optional)
+void axpby3(Ctx &cx, long long n, float a, CVol3 x, float b, Vol3 y);
+void copy3(Ctx &cx, long long n, CVol3 x, Vol3 y);
+void fill(Ctx &cx, long long n, float *y, float v);
+void scale_to(Ctx &cx, long long n, const float *x, float s, float *y);
+
+}  // namespace lam

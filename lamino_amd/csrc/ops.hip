@@ -1,0 +1,368 @@
+// Fused elementwise, stencil, reduction and preconditioner kernels.
+#include <cmath>
+
+#include "fft.h"
+#include "ops.h"
+
+namespace lam {
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+struct RedSpec {
+    RedTerm t[kMaxRedTerms];
+    int nterms;
+    long long n;
+};
+
+__global__ __launch_bounds__(256) void k_reduce(RedSpec s, double *partials) {
+    double acc[kMaxRedTerms];
+#pragma unroll
+    for (int q = 0; q < kMaxRedTerms; ++q) acc[q] = 0.0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < s.n; i += stride) {
+#pragma unroll
+        for (int q = 0; q < kMaxRedTerms; ++q) {
+            if (q < s.nterms) {
+                const RedTerm &t = s.t[q];
+                const double a = t.a[i];
+                if (t.kind == RED_DOT) acc[q] += a * (double)t.b[i];
+                else if (t.kind == RED_DIFF2) { const double d = a - (double)t.b[i]; acc[q] += d * d; }
+                else acc[q] += a * a;
+            }
+        }
+    }
+    for (int q = 0; q < s.nterms; ++q) {
+        const double v = block_sum_256(acc[q]);
+        if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// fixed-order final sum of the partials of each term
+__global__ __launch_bounds__(256) void k_reduce_final(const double *partials, int nblocks, int nterms, double *out) {
+    for (int q = 0; q < nterms; ++q) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partials[(size_t)q * nblocks + b];
+        const double v = block_sum_256(acc);
+        if (threadIdx.x == 0) out[q] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_max_partial(const float *a, long long n, float *partials) {
+    __shared__ float sh[256];
+    float m = -INFINITY;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        m = fmaxf(m, a[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// ------------------------------------------------------------------ stencils
+__device__ __forceinline__ bool interior(int i, int j, int k, int n1, int n2, int n3) {
+    return i >= 1 && i < n1 - 1 && j >= 1 && j < n2 - 1 && k >= 1 && k < n3 - 1;
+}
+
+__global__ __launch_bounds__(256) void k_grad_u(const float *__restrict__ u, float *g0, float *g1, float *g2, int n1,
+                                                int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s1 = (long long)n2 * n3, s2 = n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        float a = 0.f, b = 0.f, c = 0.f;
+        if (interior(i, j, k, n1, n2, n3)) {
+            a = (u[id + s1] - u[id - s1]) * 0.5f;
+            b = (u[id + s2] - u[id - s2]) * 0.5f;
+            c = (u[id + 1] - u[id - 1]) * 0.5f;
+        }
+        g0[id] = a; g1[id] = b; g2[id] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_divergence(const float *__restrict__ v0, const float *__restrict__ v1,
+                                                    const float *__restrict__ v2, float *out, int n1, int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s1 = (long long)n2 * n3, s2 = n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        float r = 0.f;
+        if (interior(i, j, k, n1, n2, n3)) {
+            const float A = v0[id + s1] - v0[id - s1];
+            const float B = v1[id + s2] - v1[id - s2];
+            const float C = v2[id + 1] - v2[id - 1];
+            r = (float)(-(double)A / 2.0 - (double)B / 2.0 - (double)C / 2.0);
+        }
+        out[id] = r;
+    }
+}
+
+// grad_u component along an axis at a point that is displaced by +-1 along it
+// from an interior voxel: nonzero only when that point is itself interior.
+__device__ __forceinline__ float lap_at(const float *u, long long id, int i, int j, int k, int n1, int n2, int n3,
+                                        long long s1, long long s2) {
+    // axis 0 (n1): g0(i+1) - g0(i-1)
+    const float g0p = (i + 1 < n1 - 1) ? (u[id + 2 * s1] - u[id]) * 0.5f : 0.f;
+    const float g0m = (i - 1 >= 1) ? (u[id] - u[id - 2 * s1]) * 0.5f : 0.f;
+    const float g1p = (j + 1 < n2 - 1) ? (u[id + 2 * s2] - u[id]) * 0.5f : 0.f;
+    const float g1m = (j - 1 >= 1) ? (u[id] - u[id - 2 * s2]) * 0.5f : 0.f;
+    const float g2p = (k + 1 < n3 - 1) ? (u[id + 2] - u[id]) * 0.5f : 0.f;
+    const float g2m = (k - 1 >= 1) ? (u[id] - u[id - 2]) * 0.5f : 0.f;
+    const float A = g0p - g0m, B = g1p - g1m, C = g2p - g2m;
+    return (float)(-(double)A / 2.0 - (double)B / 2.0 - (double)C / 2.0);
+}
+
+__global__ __launch_bounds__(256) void k_laplacian(const float *__restrict__ u, float *out, int n1, int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s1 = (long long)n2 * n3, s2 = n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        out[id] = interior(i, j, k, n1, n2, n3) ? lap_at(u, id, i, j, k, n1, n2, n3, s1, s2) : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add_mu_lap3(CVol3 x, Vol3 y, float mu, int n1, int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s1 = (long long)n2 * n3, s2 = n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        if (!interior(i, j, k, n1, n2, n3)) continue;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y.c[c][id] += lap_at(x.c[c], id, i, j, k, n1, n2, n3, s1, s2) * mu;
+    }
+}
+
+// ------------------------------------------------------------------ qGGMRF
+__constant__ float c_qw[27] = {0.0302f, 0.037f, 0.0302f, 0.037f, 0.0523f, 0.037f, 0.0302f, 0.037f, 0.0302f,
+                               0.037f,  0.0523f, 0.037f, 0.0532f, 0.f,    0.0523f, 0.037f, 0.0523f, 0.037f,
+                               0.0302f, 0.037f, 0.0302f, 0.037f, 0.0523f, 0.037f, 0.0302f, 0.037f, 0.0302f};
+
+// d_potential_fcn with the reference's mixed float/double arithmetic
+// (src/qggmrf.cpp:41-57): pow in double, the rest as written there.
+__device__ __forceinline__ float dpot(float delta, float sigma_q, float sigma_qp, float p) {
+    const double Q = 2.0;
+    const float C = 0.001f;
+    const float ad = fabsf(delta);
+    if (ad > 0.f) {
+        const float t1 = (float)(pow((double)ad, Q - (double)p) / (double)sigma_qp);
+        const float t3 = C + t1;
+        return (float)((double)(delta / (t3 * sigma_q)) * (Q - ((Q - (double)p) * (double)t1) / (double)t3));
+    }
+    return (float)(Q / ((double)sigma_q * 0.001));
+}
+
+__global__ __launch_bounds__(256) void k_qggmrf(float *g, const float *__restrict__ x, int n1, int n2, int n3,
+                                                float sigma_q, float sigma_qp, float p) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s1 = (long long)n2 * n3, s2 = n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        if (!interior(i, j, k, n1, n2, n3)) continue;
+        const float u = x[id];
+        float sum = 0.f;
+#pragma unroll
+        for (int a = -1; a < 2; ++a)
+#pragma unroll
+            for (int b = -1; b < 2; ++b)
+#pragma unroll
+                for (int c = -1; c < 2; ++c) {
+                    if (a == 0 && b == 0 && c == 0) continue;
+                    const float v = x[id + a * s1 + b * s2 + c];
+                    sum += c_qw[(a + 1) * 9 + (b + 1) * 3 + (c + 1)] * dpot(v - u, sigma_q, sigma_qp, p);
+                }
+        g[id] += sum;
+    }
+}
+
+// ------------------------------------------------------------------ preconditioner filter
+// Full-spectrum ramp |k| of the half-spectrum filter of include/precond.h:41-61
+// (the c2r Hermitian extension makes it even in both axes).
+__global__ __launch_bounds__(256) void k_ramp_filter(float2 *a, float2 *b, int n1, int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2);
+        const int kh = k <= n3 / 2 ? k : n3 - k;
+        const float xf = (float)kh / (float)n3;
+        const float yf = (j <= n2 / 2) ? (float)j / (float)n2 : (float)(j - n2) / (float)n2;
+        const float f = sqrtf(xf * xf + yf * yf);
+        a[id] = cscale(a[id], f);
+        if (b) b[id] = cscale(b[id], f);
+    }
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ __launch_bounds__(256) void k_axpby3(long long n, float a, CVol3 x, float b, Vol3 y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (y.c[c]) y.c[c][i] = a * x.c[c][i] + b * y.c[c][i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy3(long long n, CVol3 x, Vol3 y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (y.c[c]) y.c[c][i] = x.c[c][i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill(long long n, float *y, float v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_scale_to(long long n, const float *x, float s, float *y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i] * s;
+}
+
+unsigned vgrid(long long n) { return grid_for((size_t)n, 256, 1u << 16); }
+
+}  // namespace
+
+void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev) {
+    LAM_REQUIRE(nterms >= 1 && nterms <= kMaxRedTerms, LAM_ERR_ARG, "bad reduction");
+    RedSpec s;
+    for (int q = 0; q < nterms; ++q) s.t[q] = terms[q];
+    for (int q = nterms; q < kMaxRedTerms; ++q) s.t[q] = RedTerm{RED_SQ, nullptr, nullptr};
+    s.nterms = nterms;
+    s.n = n;
+    cx.red.ensure(sizeof(double) * kRedBlocks * kMaxRedTerms);
+    hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, cx.stream, s, cx.red.as<double>());
+    check_launch();
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, cx.stream, cx.red.as<double>(), kRedBlocks, nterms,
+                       out_dev);
+    check_launch();
+}
+
+void reduce_terms(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_host) {
+    cx.scal.ensure(sizeof(double) * 64);
+    reduce_terms_dev(cx, n, terms, nterms, cx.scal.as<double>());
+    LAM_HIP(hipMemcpyAsync(out_host, cx.scal.ptr, sizeof(double) * nterms, hipMemcpyDeviceToHost, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+}
+
+float max_value(Ctx &cx, long long n, const float *a) {
+    const int nb = 256;
+    cx.red.ensure(sizeof(double) * kRedBlocks * kMaxRedTerms);
+    hipLaunchKernelGGL(k_max_partial, dim3(nb), dim3(256), 0, cx.stream, a, n, reinterpret_cast<float *>(cx.red.ptr));
+    check_launch();
+    std::vector<float> h(nb);
+    LAM_HIP(hipMemcpyAsync(h.data(), cx.red.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+    float m = -INFINITY;
+    for (float v : h) m = std::fmax(m, v);
+    return m;
+}
+
+void grad_u(Ctx &cx, const long long d[3], const float *u, float *g0, float *g1, float *g2) {
+    const long long n = d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_grad_u, dim3(vgrid(n)), dim3(256), 0, cx.stream, u, g0, g1, g2, (int)d[0], (int)d[1],
+                       (int)d[2]);
+    check_launch();
+}
+
+void divergence(Ctx &cx, const long long d[3], const float *v0, const float *v1, const float *v2, float *out) {
+    const long long n = d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_divergence, dim3(vgrid(n)), dim3(256), 0, cx.stream, v0, v1, v2, out, (int)d[0], (int)d[1],
+                       (int)d[2]);
+    check_launch();
+}
+
+void laplacian(Ctx &cx, const long long d[3], const float *u, float *out) {
+    const long long n = d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_laplacian, dim3(vgrid(n)), dim3(256), 0, cx.stream, u, out, (int)d[0], (int)d[1], (int)d[2]);
+    check_launch();
+}
+
+void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu) {
+    const long long n = d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_add_mu_lap3, dim3(vgrid(n)), dim3(256), 0, cx.stream, x, y, mu, (int)d[0], (int)d[1],
+                       (int)d[2]);
+    check_launch();
+}
+
+void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma, float p) {
+    const long long n = d[0] * d[1] * d[2];
+    const float sq = (float)std::pow((double)sigma, 2.0);
+    const float sqp = (float)std::pow((double)sigma, 2.0 - (double)p);
+    hipLaunchKernelGGL(k_qggmrf, dim3(vgrid(n)), dim3(256), 0, cx.stream, g, x, (int)d[0], (int)d[1], (int)d[2], sq,
+                       sqp, p);
+    check_launch();
+}
+
+void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1) {
+    const int n1 = (int)d[0], n2 = (int)d[1], n3 = (int)d[2];
+    const long long n = d[0] * d[1] * d[2];
+    cx.work[2].ensure(sizeof(float2) * (size_t)n);
+    float2 *a = cx.work[2].as<float2>();
+    // rows (n3), real or real-pair in
+    {
+        AxisPass p;
+        p.in = in0; p.in2 = in1;
+        p.ld_mode = in1 ? LD_REAL_PAIR : LD_REAL;
+        p.is1 = (long long)n2 * n3; p.is2 = n3; p.isn = 1;
+        p.out = a; p.os1 = (long long)n2 * n3; p.os2 = n3; p.osn = 1;
+        p.c1 = n1; p.c2 = n2; p.ci = 1;
+        run_axis(cx, get_fft_plan(cx, n3, -1), p);
+    }
+    {
+        AxisPass p;
+        p.in = a; p.out = a;
+        p.is1 = p.os1 = (long long)n2 * n3; p.isi = p.osi = 1; p.isn = p.osn = n3;
+        p.c1 = n1; p.c2 = 1; p.ci = n3;
+        run_axis(cx, get_fft_plan(cx, n2, -1), p);
+    }
+    hipLaunchKernelGGL(k_ramp_filter, dim3(vgrid(n)), dim3(256), 0, cx.stream, a, (float2 *)nullptr, n1, n2, n3);
+    check_launch();
+    {
+        AxisPass p;
+        p.in = a; p.out = a;
+        p.is1 = p.os1 = (long long)n2 * n3; p.isi = p.osi = 1; p.isn = p.osn = n3;
+        p.c1 = n1; p.c2 = 1; p.ci = n3;
+        run_axis(cx, get_fft_plan(cx, n2, +1), p);
+    }
+    {
+        AxisPass p;
+        p.in = a; p.is1 = (long long)n2 * n3; p.is2 = n3; p.isn = 1;
+        p.out = out0; p.out2 = out1;
+        p.st_mode = out1 ? ST_REAL_PAIR : ST_REAL;
+        p.os1 = (long long)n2 * n3; p.os2 = n3; p.osn = 1;
+        p.c1 = n1; p.c2 = n2; p.ci = 1;
+        p.scale = (float)(1.0 / ((double)n2 * (double)n3));
+        run_axis(cx, get_fft_plan(cx, n3, +1), p);
+    }
+}
+
+void axpby3(Ctx &cx, long long n, float a, CVol3 x, float b, Vol3 y) {
+    hipLaunchKernelGGL(k_axpby3, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, a, x, b, y);
+    check_launch();
+}
+
+void copy3(Ctx &cx, long long n, CVol3 x, Vol3 y) {
+    hipLaunchKernelGGL(k_copy3, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, x, y);
+    check_launch();
+}
+
+void fill(Ctx &cx, long long n, float *y, float v) {
+    hipLaunchKernelGGL(k_fill, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, y, v);
+    check_launch();
+}
+
+void scale_to(Ctx &cx, long long n, const float *x, float s, float *y) {
+    hipLaunchKernelGGL(k_scale_to, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, x, s, y);
+    check_launch();
+}
+
+}  // namespace lam

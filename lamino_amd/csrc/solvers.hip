@@ -1,0 +1,302 @@
+// Device-resident PCG, split-Bregman and Nesterov solvers and the MBIR drivers.
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <random>
+
+#include "solvers.h"
+
+namespace lam {
+
+namespace {
+
+unsigned vgrid(long long n) { return grid_for((size_t)n, 256, 1u << 16); }
+
+// x += a p ; r -= a Ap  (3 components)
+__global__ __launch_bounds__(256) void k_cg_step(long long n, float alpha, Vol3 x, Vol3 r, CVol3 p, CVol3 Ap) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x.c[c][i] += p.c[c][i] * alpha;
+            r.c[c][i] -= Ap.c[c][i] * alpha;
+        }
+    }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_cg_dir(long long n, float beta, CVol3 z, Vol3 p) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.c[c][i] = z.c[c][i] + p.c[c][i] * beta;
+    }
+}
+
+// r = b - Ax
+__global__ __launch_bounds__(256) void k_sub3(long long n, CVol3 b, CVol3 Ax, Vol3 r) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r.c[c][i] = b.c[c][i] - Ax.c[c][i];
+    }
+}
+
+struct Vol9 {
+    float *c[9];
+};
+
+__device__ __forceinline__ bool interior(int i, int j, int k, int n1, int n2, int n3) {
+    return i >= 1 && i < n1 - 1 && j >= 1 && j < n2 - 1 && k >= 1 && k < n3 - 1;
+}
+
+// rhs_i = yT_i + mu * divergence(d_i - b_i)   (src/bregman.cpp:81-84)
+__global__ __launch_bounds__(256) void k_sb_rhs(CVol3 yT, Vol9 d, Vol9 b, Vol3 rhs, float mu, int n1, int n2,
+                                                int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s[3] = {(long long)n2 * n3, n3, 1};
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s[0]);
+        const bool in = interior(i, j, k, n1, n2, n3);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float dv = 0.f;
+            if (in) {
+                float A[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float *dd = d.c[3 * c + a], *bb = b.c[3 * c + a];
+                    const float vp = dd[id + s[a]] - bb[id + s[a]];
+                    const float vm = dd[id - s[a]] - bb[id - s[a]];
+                    A[a] = vp - vm;
+                }
+                dv = (float)(-(double)A[0] / 2.0 - (double)A[1] / 2.0 - (double)A[2] / 2.0);
+            }
+            rhs.c[c][id] = yT.c[c][id] + dv * mu;
+        }
+    }
+}
+
+// isotropic shrinkage + Bregman update (src/bregman.cpp:91-120), gradients of x recomputed in place
+__global__ __launch_bounds__(256) void k_sb_shrink(CVol3 x, Vol9 d, Vol9 b, float eps, float thr, int n1, int n2,
+                                                   int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const long long s[3] = {(long long)n2 * n3, n3, 1};
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s[0]);
+        const bool in = interior(i, j, k, n1, n2, n3);
+        float g[9], t[9];
+        float sk = eps;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float gv = in ? (x.c[c][id + s[a]] - x.c[c][id - s[a]]) * 0.5f : 0.f;
+                g[3 * c + a] = gv;
+                const float tv = gv + b.c[3 * c + a][id];
+                t[3 * c + a] = tv;
+                sk += tv * tv;
+            }
+        sk = sqrtf(sk);
+        const float f = fmaxf(sk - thr, 0.f);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const float dn = f * (t[q] / sk);
+            d.c[q][id] = dn;
+            b.c[q][id] += g[q] - dn;
+        }
+    }
+}
+
+// y = x + (x - xold) * beta   (src/nagopt.cpp:156-158)
+__global__ __launch_bounds__(256) void k_nag_y(long long n, float beta, CVol3 x, CVol3 xold, Vol3 y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y.c[c][i] = x.c[c][i] + (x.c[c][i] - xold.c[c][i]) * beta;
+    }
+}
+
+// x = y - g * step   (src/nagopt.cpp:162)
+__global__ __launch_bounds__(256) void k_nag_x(long long n, float step, CVol3 y, CVol3 g, Vol3 x) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x.c[c][i] = y.c[c][i] - g.c[c][i] * step;
+    }
+}
+
+}  // namespace
+
+void cg_solve(Ctx &cx, const long long d[3], const OpFn &A, CVol3 b, CVol3 x0, Vol3 x, long long max_iters,
+              double tol, CgStats *st) {
+    const long long n = d[0] * d[1] * d[2];
+    Vol3Buf r, z, p, Ap;
+    r.alloc(n); z.alloc(n); p.alloc(n); Ap.alloc(n);
+    if (x.c[0] != x0.c[0]) copy3(cx, n, x0, x);
+    A(cv(x), Ap.v());
+    long long applies = 1;
+    hipLaunchKernelGGL(k_sub3, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, b, Ap.cv(), r.v());
+    check_launch();
+    auto precond3 = [&](CVol3 in, Vol3 out) {
+        precond_apply(cx, d, in.c[0], in.c[1], out.c[0], out.c[1]);
+        precond_apply(cx, d, in.c[2], nullptr, out.c[2], nullptr);
+    };
+    auto dot3 = [&](CVol3 a, CVol3 bb) {
+        RedTerm t[3] = {{RED_DOT, a.c[0], bb.c[0]}, {RED_DOT, a.c[1], bb.c[1]}, {RED_DOT, a.c[2], bb.c[2]}};
+        double s[3];
+        reduce_terms(cx, n, t, 3, s);
+        return s[0] + s[1] + s[2];
+    };
+    precond3(r.cv(), z.v());
+    copy3(cx, n, z.cv(), p.v());
+    double rs_old = dot3(z.cv(), r.cv());
+    long long it = 0;
+    for (; it < max_iters; ++it) {
+        A(p.cv(), Ap.v());
+        ++applies;
+        const double pAp = dot3(p.cv(), Ap.cv());
+        if (std::fabs(pAp) < 1e-10) break;
+        const double alpha = rs_old / pAp;
+        hipLaunchKernelGGL(k_cg_step, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (float)alpha, x, r.v(), p.cv(),
+                           Ap.cv());
+        check_launch();
+        precond3(r.cv(), z.v());
+        const double rs_new = dot3(z.cv(), r.cv());
+        if (std::sqrt(rs_new) < tol) { ++it; break; }
+        const double beta = rs_new / rs_old;
+        hipLaunchKernelGGL(k_cg_dir, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (float)beta, z.cv(), p.v());
+        check_launch();
+        rs_old = rs_new;
+    }
+    if (st) { st->iters += it; st->applies += applies; }
+}
+
+void split_bregman(Ctx &cx, const long long d[3], const OpFn &A, bool A_has_lap, CVol3 yT, CVol3 x0, double lambda,
+                   double mu, long long outer_max, long long inner_max, double tol, double xtol, Vol3 x,
+                   CgStats *st, long long *outer_done) {
+    const long long n = d[0] * d[1] * d[2];
+    Vol3Buf xold, rhs;
+    xold.alloc(n); rhs.alloc(n);
+    DevBuf dd(sizeof(float) * 9 * (size_t)n), bb(sizeof(float) * 9 * (size_t)n);
+    LAM_HIP(hipMemsetAsync(dd.ptr, 0, dd.bytes, cx.stream));
+    LAM_HIP(hipMemsetAsync(bb.ptr, 0, bb.bytes, cx.stream));
+    Vol9 D, B;
+    for (int q = 0; q < 9; ++q) { D.c[q] = dd.as<float>() + q * n; B.c[q] = bb.as<float>() + q * n; }
+    copy3(cx, n, x0, xold.v());
+    if (x.c[0] != x0.c[0]) copy3(cx, n, x0, x);
+    OpFn Ap = A;
+    if (!A_has_lap) {
+        Ap = [&](CVol3 u, Vol3 y) {
+            A(u, y);
+            add_mu_laplacian3(cx, d, u, y, (float)mu);
+        };
+    }
+    const float eps = 1e-8f;
+    const float thr = (float)lambda / (float)mu;
+    long long it = 0;
+    for (; it < outer_max; ++it) {
+        hipLaunchKernelGGL(k_sb_rhs, dim3(vgrid(n)), dim3(256), 0, cx.stream, yT, D, B, rhs.v(), (float)mu, (int)d[0],
+                           (int)d[1], (int)d[2]);
+        check_launch();
+        cg_solve(cx, d, Ap, rhs.cv(), xold.cv(), x, inner_max, tol, st);
+        hipLaunchKernelGGL(k_sb_shrink, dim3(vgrid(n)), dim3(256), 0, cx.stream, cv(x), D, B, eps, thr, (int)d[0],
+                           (int)d[1], (int)d[2]);
+        check_launch();
+        RedTerm t[6];
+        for (int c = 0; c < 3; ++c) {
+            t[c] = RedTerm{RED_DIFF2, x.c[c], xold.v().c[c]};
+            t[3 + c] = RedTerm{RED_SQ, xold.v().c[c], nullptr};
+        }
+        double s[6];
+        reduce_terms(cx, n, t, 6, s);
+        double nd = 0;
+        for (int c = 0; c < 3; ++c) nd += std::sqrt(s[c]) / (std::sqrt(s[3 + c]) + 1e-8);
+        copy3(cx, n, cv(x), xold.v());
+        if (nd < xtol) { ++it; break; }
+    }
+    if (outer_done) *outer_done = it;
+}
+
+void nagopt(Ctx &cx, const long long d[3], const OpFn &grad, const LossFn &loss, Vol3 x, long long max_iters,
+            double lipschitz, double tol, double xtol, long long max_inner, long long *outer_done,
+            long long *grad_calls) {
+    const long long n = d[0] * d[1] * d[2];
+    Vol3Buf xold, y, g;
+    xold.alloc(n); y.alloc(n); g.alloc(n);
+    copy3(cx, n, cv(x), xold.v());
+    double t = 1, tnew = 1;
+    double step = 0.1 / lipschitz, step_prev = step;
+    long long iter = 0, calls = 0;
+    for (; iter < max_iters; ++iter) {
+        bool converged = false;
+        for (long long inner = 0; inner < max_inner; ++inner) {
+            const double beta = tnew * (1 / t - 1);
+            tnew = 0.5 * (std::sqrt(std::pow(t, 4) + 4 * std::pow(t, 2)) - std::pow(t, 2));
+            hipLaunchKernelGGL(k_nag_y, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (float)beta, cv(x), xold.cv(),
+                               y.v());
+            check_launch();
+            grad(y.cv(), g.v());
+            ++calls;
+            hipLaunchKernelGGL(k_nag_x, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (float)step, y.cv(), g.cv(), x);
+            check_launch();
+            const double ex = loss(cv(x));
+            const double ey = loss(y.cv());
+            RedTerm tg[3] = {{RED_SQ, g.v().c[0], nullptr}, {RED_SQ, g.v().c[1], nullptr}, {RED_SQ, g.v().c[2], nullptr}};
+            double sg[3];
+            reduce_terms(cx, n, tg, 3, sg);
+            const double gy = 0.5 * step * (std::sqrt(sg[0]) + std::sqrt(sg[1]) + std::sqrt(sg[2]));
+            if (ex > ey + gy) {
+                step *= 0.9;
+            } else {
+                step = step_prev;
+                t = tnew;
+                RedTerm te[6];
+                for (int c = 0; c < 3; ++c) {
+                    te[c] = RedTerm{RED_DIFF2, x.c[c], xold.v().c[c]};
+                    te[3 + c] = RedTerm{RED_SQ, x.c[c], nullptr};
+                }
+                double se[6];
+                reduce_terms(cx, n, te, 6, se);
+                const double xerr = (std::sqrt(se[0]) + std::sqrt(se[1]) + std::sqrt(se[2])) /
+                                    (std::sqrt(se[3]) + std::sqrt(se[4]) + std::sqrt(se[5]));
+                if (xerr < xtol) { converged = true; break; }
+                copy3(cx, n, cv(x), xold.v());
+                break;
+            }
+        }
+        if (converged) { ++iter; break; }
+        const double e = loss(cv(x));
+        if (e < tol) { ++iter; break; }
+    }
+    if (outer_done) *outer_done = iter;
+    if (grad_calls) *grad_calls = calls;
+}
+
+double lipschitz(Ctx &cx, const long long d[3], const OpFn &A, long long max_iters, double tol, uint64_t seed) {
+    const long long n = d[0] * d[1] * d[2];
+    Vol3Buf x, Ax;
+    x.alloc(n); Ax.alloc(n);
+    std::vector<float> h(3 * (size_t)n);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (auto &v : h) v = (float)U(rng);
+    LAM_HIP(hipMemcpyAsync(x.buf.ptr, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, cx.stream));
+    auto norm3 = [&](CVol3 a) {
+        RedTerm t[3] = {{RED_SQ, a.c[0], nullptr}, {RED_SQ, a.c[1], nullptr}, {RED_SQ, a.c[2], nullptr}};
+        double s[3];
+        reduce_terms(cx, n, t, 3, s);
+        return std::sqrt(s[0] + s[1] + s[2]);
+    };
+    double nx = norm3(x.cv());
+    axpby3(cx, n, (float)(1.0 / nx), x.cv(), 0.f, x.v());
+    double L = 0;
+    for (long long i = 0; i < max_iters; ++i) {
+        A(x.cv(), Ax.v());
+        const double na = norm3(Ax.cv());
+        if (std::fabs(na) < 1e-12) { L = 0; break; }
+        axpby3(cx, n, (float)(1.0 / na), Ax.cv(), 0.f, x.v());
+        if (std::fabs(na - L) < tol) break;   // returns the previous estimate (src/lipschitz.cpp:50-51)
+        L = na;
+    }
+    return L;
+}
+
+}  // namespace lam

@@ -1,0 +1,39 @@
+// Toeplitz form of the normal operator sum_g sysmat_g (src/gradient.cpp:33-89).
+//
+// sysmat_g(x)_i[m'] = sum_k sum_m T^g_ik(m' - m) x_k[m] with the exact kernel
+//   T^g_ik(D) = 1/(R C) sum_p d_i(p) d_k(p) Dir_C(2pi/C D.u_p) Dir_R(2pi/R D.v_p),
+//   Dir_n(t) = sin(n t / 2) / sin(t / 2),  D in (-n, n)^3,
+// so the operator is a 3x3 block convolution evaluated with zero-padded FFTs on
+// an L-grid (L_a >= 2 n_a - 1, 2,3,5-smooth).  The kernels are real and even,
+// hence their spectra are real: pairs of them are stored as one complex array.
+#pragma once
+#include "common.h"
+#include "nufft.h"
+#include "ops.h"
+
+namespace lam {
+
+struct Toep {
+    Ctx *cx = nullptr;
+    long long n[3] = {0, 0, 0};
+    int L[3] = {0, 0, 0};
+    int ngrids = 0;
+    bool per_grid = false;
+    DevBuf khat;                    // float2 [L1][L2][L3][3]: (K00,K01),(K02,K11),(K12,K22) spectra / Ltot
+    std::vector<DevBuf> khat_g;     // per-grid spectra (when per_grid)
+    DevBuf za, zb, wa, wb;          // work spectra (complex L-grid each)
+    long long applies = 0;
+};
+
+std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], bool per_grid);
+// y = sum_g sysmat_g(x) + mu * laplacian(x)
+void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu);
+// q[g*3+i] = x_i . sysmat_g(x)_i  (device FP64 array of 3*ngrids; no sync)
+void toep_quad_dev(Toep &t, CVol3 x, double *q_dev);
+void toep_quad(Toep &t, CVol3 x, double *q_host);
+
+}  // namespace lam
+
+struct lam_toep {
+    std::unique_ptr<lam::Toep> impl;
+};

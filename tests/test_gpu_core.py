@@ -1,0 +1,167 @@
+"""GPU parity of the core operators against the CPU oracle (small shapes).
+
+Tolerances: forward/adjoint/NUFFT results within 1e-4 relative L2 of the FP64
+oracle (NUFFT tolerance 1e-9); FFT, stencils and preconditioner at FP32
+round-off.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel, say
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-4
+
+
+@pytest.mark.parametrize("shape", [(2, 7, 9), (3, 89, 89), (2, 43, 179), (1, 305, 305), (2, 90, 360), (1, 21, 45),
+                                   (1, 8, 10), (2, 1, 16), (1, 615, 17)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft2_matches_numpy(ctx, shape, sign):
+    import lamino_amd as la
+
+    say("fft2", shape, sign)
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    y = la.fft2(x, sign)
+    ref = np.fft.fft2(x.astype(np.complex128)) if sign < 0 else np.fft.ifft2(x.astype(np.complex128)) * shape[1] * shape[2]
+    assert rel(y, ref) < 5e-6
+
+
+@pytest.mark.parametrize("shape", [(3, 7, 9), (2, 8, 10)])
+def test_fft2_shifted_matches_reference_composition(ctx, orc, shape):
+    import lamino_amd as la
+
+    rng = np.random.default_rng(2)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    for sign in (-1, 1):
+        want = orc.shift2(x, inverse=False)
+        want = np.fft.fft2(want) if sign < 0 else np.fft.ifft2(want) * shape[1] * shape[2]
+        want = orc.shift2(want, inverse=True)
+        assert rel(la.fft2(x, sign, shifted=True), want) < 5e-6
+
+
+@pytest.mark.parametrize("dims", [(5, 7, 9), (6, 8, 10), (4, 21, 89), (3, 43, 179)])
+def test_precond_matches_oracle(ctx, orc, dims):
+    import lamino_amd as la
+
+    say("precond", dims)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(dims).astype(np.float32)
+    y = la.RampPreconditioner(dims).apply(x)
+    assert rel(y, orc.precond(x)) < 5e-6
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 7), (9, 12, 11)])
+def test_stencils_match_oracle(ctx, orc, dims):
+    import lamino_amd as la
+
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(dims).astype(np.float32)
+    v = rng.standard_normal((3,) + dims).astype(np.float32)
+    np.testing.assert_array_equal(la.grad_u(u), orc.grad_u(u, prec=32).astype(np.float32))
+    np.testing.assert_array_equal(la.divergence(v), orc.divergence(v, prec=32).astype(np.float32))
+    lap_ref = orc.divergence(orc.grad_u(u, prec=32).astype(np.float32), prec=32).astype(np.float32)
+    np.testing.assert_array_equal(la.laplacian(u), lap_ref)
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 7), (9, 12, 11)])
+def test_qggmrf_matches_oracle(ctx, orc, dims):
+    import lamino_amd as la
+
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(dims).astype(np.float32)
+    x[2, 2, 2] = x[2, 2, 3]  # delta = 0 branch
+    g = rng.standard_normal(dims).astype(np.float32)
+    want = orc.qggmrf(g, x, 1000.0, 1.2, prec=32)
+    got = la.qggmrf(g.copy(), x, 1000.0, 1.2)
+    assert np.max(np.abs(got - want)) <= 2e-6 * max(1.0, np.max(np.abs(want)))
+
+
+def _geom(P, seed=7):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, np.pi, P), rng.uniform(0, 2 * np.pi, P)
+
+
+@pytest.mark.parametrize("dims,R,C,P", [((5, 3, 7), 7, 9, 4), ((6, 4, 8), 8, 9, 3)])
+def test_nufft_tiny_vs_brute_force(ctx, orc, dims, R, C, P):
+    import lamino_amd as la
+
+    say("nufft tiny", dims, R, C, P)
+    al, ga = _geom(P)
+    pg = la.PolarGrid.per_projection(al, ga, R, C)
+    rng = np.random.default_rng(8)
+    f = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims))
+    c = la.nufft3d2(f, pg)
+    assert rel(c, orc.nufft(2, dims, al, ga, R, C, f, brute=True)) < FWD_TOL
+    cin = rng.standard_normal((P, R, C)) + 1j * rng.standard_normal((P, R, C))
+    fo = la.nufft3d1(cin, dims, pg)
+    assert rel(fo, orc.nufft(1, dims, al, ga, R, C, cin, brute=True)) < FWD_TOL
+
+
+@pytest.mark.parametrize("literal", [True, False])
+def test_forward_adjoint_vs_oracle(ctx, orc, literal):
+    import lamino_amd as la
+
+    dims, R, C, P = (9, 5, 11), 13, 15, 12
+    rng = np.random.default_rng(9)
+    if literal:
+        th = np.linspace(-1.2, 0.0, P)
+        pg = la.PolarGrid(th, R, C, 0.6)
+        al, ga = th, np.full(P, 0.6)
+    else:
+        al, ga = np.full(P, math.radians(35)), np.linspace(0, 2 * np.pi, P, endpoint=False) + 0.3
+        pg = la.PolarGrid.per_projection(al, ga, R, C)
+    m = rng.standard_normal((3,) + dims).astype(np.float32)
+    y = la.forward(m, pg)
+    yo = orc.forward(m, al, ga, R, C)
+    say("forward rel", rel(y, yo))
+    assert rel(y, yo) < FWD_TOL
+    p = rng.standard_normal((P, R, C)).astype(np.float32)
+    a = la.adjoint(p, pg, dims)
+    ao = orc.adjoint(p, dims, al, ga)
+    say("adjoint rel", rel(a, ao))
+    assert rel(a, ao) < FWD_TOL
+
+
+def test_sysmat_and_toeplitz_vs_oracle(ctx, orc):
+    import lamino_amd as la
+
+    dims, R, C, P = (9, 5, 11), 13, 15, 10
+    al1, ga1 = np.full(P, 0.5), np.linspace(0, 2 * np.pi, P, endpoint=False)
+    al2, ga2 = np.linspace(-1, 1, P), np.full(P, 0.8)
+    g1 = la.PolarGrid.per_projection(al1, ga1, R, C)
+    g2 = la.PolarGrid.per_projection(al2, ga2, R + 2, C + 4)
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal((3,) + dims).astype(np.float32)
+    s1 = la.sysmat(x, g1)
+    so1 = orc.sysmat(x, [(al1, ga1, R, C)])
+    say("sysmat rel", rel(s1, so1))
+    assert rel(s1, so1) < FWD_TOL
+    T = la.ToeplitzOperator([g1, g2], dims, keep_per_grid=True)
+    so = orc.sysmat(x, [(al1, ga1, R, C), (al2, ga2, R + 2, C + 4)])
+    t = T(x)
+    say("toeplitz rel", rel(t, so))
+    assert rel(t, so) < 2e-5
+    q = T.quad(x)
+    so2 = orc.sysmat(x, [(al2, ga2, R + 2, C + 4)])
+    want = np.array([[np.vdot(x[i].astype(np.float64), so1[i]) for i in range(3)],
+                     [np.vdot(x[i].astype(np.float64), so2[i]) for i in range(3)]])
+    assert np.max(np.abs(q - want)) / np.max(np.abs(want)) < 2e-5
+
+
+def test_cg_diag_matches_oracle(ctx, orc):
+    """tests/test_cg.cpp:16-54 pattern: diagonal operator through the host-callback adapter."""
+    import lamino_amd as la
+
+    dims = (4, 5, 6)
+    rng = np.random.default_rng(11)
+    diag = (1.0 + np.arange(3 * np.prod(dims)).reshape((3,) + dims) / 10.0).astype(np.float32)
+    b = rng.standard_normal((3,) + dims).astype(np.float32)
+    x0 = np.zeros_like(b)
+    x, it = la.cgsolver(lambda v: diag * v, b, x0, 25, 0.0)
+    xo, ito = orc.cg_diag(diag, b, x0, 25, 0.0)
+    assert it == ito
+    assert rel(x, xo) < 1e-4
