@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the laminography reconstruction hot path on one MI355X.
+
+Workload (split-Bregman configuration): volume [32,128,128] x 3 -> padded
+(179, 43, 179); detector 128 -> 127 -> padded 305 x 305; two gamma datasets
+(0, 45 deg) of 180 projections; tilt 35 deg.  A step is one forward plus one
+adjoint projection of every dataset (all 3 magnetisation components), with
+the inputs resident in HBM.  Also reported: CG iterations/s of the Toeplitz
+normal operator + ramp preconditioner at the same shape, the plain-form
+reconstruction latency of the whole split-Bregman job (30 outer x 3 inner),
+the roofline of the dominant kernel and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "fwd+adj NUFFT projections/s, CG iters/s vs plain form; p50/p99 recon latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cg-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-recon", action="store_true", help="skip the full reconstruction latency")
+    ap.add_argument("--cpu-projections", type=int, default=180)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported on this build "
+              "(the path shards across GPUs by design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import lamino_amd as la
+    from lamino_amd import _lib, synth
+    from lamino_amd.api import _dims, _ptrs
+
+    work = synth.config(2)
+    vol, det = work.solver_shapes()           # (179, 43, 179), 305
+    G = len(work.scans)
+    P = len(work.scans[0].alpha)
+    log(f"workload: {work.name}; padded volume {vol}, detector {det}x{det}, {G} x {P} projections")
+
+    ctx = la.default_context()
+    lib = ctx.lib
+    eps = la.api.DEFAULT_EPS
+
+    # inputs: the padded vortex field and one projection stack per dataset, all resident on the device
+    m = np.zeros((3,) + vol, np.float32)
+    rd = work.recon_dims
+    o = [(v - n) // 2 for v, n in zip(vol, rd)]
+    m[:, o[0]:o[0] + rd[0], o[1]:o[1] + rd[1], o[2]:o[2] + rd[2]] = synth.vortex_field(rd, work.seed)
+    md = ctx.to_device(m)
+    grids = [la.PolarGrid.per_projection(s.alpha, s.gamma, det, det) for s in work.scans]
+    projs = [ctx.alloc((P, det, det), np.float32) for _ in range(G)]
+    outs = [ctx.alloc((3,) + vol, np.float32) for _ in range(G)]
+    dims = _dims(vol)
+
+    def step():
+        for g in range(G):
+            a, b, c = md.comps()
+            _lib.check(lib.lam_forward(ctx.h, grids[g].h, dims, eps, a, b, c, projs[g].ptr))
+            x, y, z = outs[g].comps()
+            _lib.check(lib.lam_adjoint(ctx.h, grids[g].h, dims, eps, projs[g].ptr, x, y, z))
+
+    log("warmup (includes per-grid point binning)")
+    for _ in range(max(args.warmup, 1)):
+        step()
+    ctx.sync()
+    _lib.check(lib.lam_timing_enable(ctx.h, 1))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = G * P / ((t1 - t0) / args.steps)
+    log(f"fwd+adj: {ms_per_step:.3f} ms/step -> {value:.1f} projections/s")
+
+    kernels = {}
+    for name in ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points"):
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(lib.lam_timing_query(ctx.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt)))
+        if cnt.value:
+            kernels[name] = {"total_ms": tot.value, "launches": cnt.value, "avg_ms": tot.value / cnt.value}
+    _lib.check(lib.lam_timing_enable(ctx.h, 0))
+    log("kernel time per step (ms):", {k: round(v["total_ms"] / args.steps, 3) for k, v in kernels.items()})
+
+    # roofline of the dominant NUFFT kernel: algorithmic bytes per launch M*(12+24) + Nfine*24
+    # (coordinates + 3 complex components per point, fine grid touched once)
+    nf = [2 * n for n in vol]
+    from lamino_amd.synth import pad_n  # noqa: F401  (shape rules live there)
+
+    def smooth_even(n):
+        while True:
+            r = n
+            for p_ in (2, 3, 5):
+                while r % p_ == 0:
+                    r //= p_
+            if r == 1 and n % 2 == 0:
+                return n
+            n += 1
+
+    nf = [smooth_even(max(2 * n, 12)) for n in vol]
+    Nfine = nf[0] * nf[1] * nf[2]
+    M = P * det * det
+    bytes_alg = M * 36 + Nfine * 24
+    dom = max(("interp", "spread"), key=lambda k: kernels.get(k, {"total_ms": 0})["total_ms"])
+    avg_s = kernels[dom]["avg_ms"] / 1e3
+    achieved = bytes_alg / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": bytes_alg, "avg_launch_ms": round(kernels[dom]["avg_ms"], 4)}
+
+    # CG iterations/s: Toeplitz normal operator + ramp preconditioner at the same shape
+    log("building Toeplitz normal operator")
+    tb0 = time.perf_counter()
+    T = la.ToeplitzOperator(grids, vol)
+    ctx.sync()
+    toep_build_s = time.perf_counter() - tb0
+    b = outs[0]
+    xd = ctx.alloc((3,) + vol, np.float32)
+    x0 = ctx.alloc((3,) + vol, np.float32)
+    x0.zero()
+    op = _lib.OP_FN(ctypes.cast(lib.lam_toep_op, ctypes.c_void_p).value)
+    it = ctypes.c_int64()
+    _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), 2, 0.0,
+                                _ptrs(xd.comps()), ctypes.byref(it)))
+    ctx.sync()
+    tc0 = time.perf_counter()
+    _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), args.cg_iters, 0.0,
+                                _ptrs(xd.comps()), ctypes.byref(it)))
+    ctx.sync()
+    cg_s = time.perf_counter() - tc0
+    cg_rate = it.value / cg_s
+    log(f"CG: {it.value} iterations in {cg_s:.3f} s -> {cg_rate:.1f} iters/s (Toeplitz build {toep_build_s:.3f} s)")
+
+    # plain-form reconstruction latency of the whole split-Bregman job
+    recon = None
+    if not args.no_recon:
+        log("simulating measured projections and running the split-Bregman job")
+
+        def project(mp, scan, dp):
+            pg = la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp)
+            return la.forward(mp, pg)
+
+        datasets, _ = synth.simulate(work, project)
+        prm = la.ReconParams(max_iters=work.params["max_iters"], inner_iters=work.params["inner_iters"],
+                             lam=work.params["lam"], mu=work.params["mu"], tol=0.0, xtol=0.0)
+        tr0 = time.perf_counter()
+        rec, st = la.MBIR2(datasets, rd, prm, stats=True)
+        tr = time.perf_counter() - tr0
+        recon = {"recon_s": round(tr, 4), "setup_s": round(st["setup_s"], 4), "solve_s": round(st["solve_s"], 4),
+                 "normal_ops": st["normal_ops"], "cg_iters": st["cg_iters"], "outer_iters": st["outer_iters"],
+                 "finite": bool(np.isfinite(rec).all())}
+        log("recon:", recon)
+
+    # CPU baseline: the oracle's FP32 restatement of the reference path (w=7, sigma=2 ES NUFFT,
+    # per-component type-1/type-2, single-threaded detector FFTs), all host cores
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        oracle.set_threads(cores)
+        sc = work.scans[0]
+        Ps = min(args.cpu_projections, P)
+        log(f"CPU oracle baseline on {cores} threads: forward+adjoint of {Ps} projections")
+        tcp0 = time.perf_counter()
+        pc = oracle.forward(m, sc.alpha[:Ps], sc.gamma[:Ps], det, det, eps=1.2e-6, prec=32)
+        oracle.adjoint(pc, vol, sc.alpha[:Ps], sc.gamma[:Ps], eps=1.2e-6, prec=32)
+        tcpu = time.perf_counter() - tcp0
+        gp = projs[0].download()[:Ps]
+        diff = float(np.linalg.norm(gp - pc) / np.linalg.norm(pc))
+        cpu = {"value": round(Ps / tcpu, 3), "unit": "projections/s", "cores": cores, "kind": "port",
+               "sample": f"forward+adjoint of {Ps} projections of the gamma=0 dataset at the padded shapes "
+                         f"(FP32, ES width 7), {tcpu:.2f} s",
+               "gpu_vs_cpu_forward_rel_l2": diff}
+        log("cpu:", cpu)
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "split-Bregman config: vol [32,128,128]x3 -> padded (179,43,179), det 128 -> 305x305, "
+                               "gammas (0,45) x 180 projections, tilt 35 deg; step = forward + adjoint of all datasets",
+                   "padded_volume": list(vol), "detector": det, "datasets": G, "projections": P,
+                   "nufft_eps": eps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "cg_iters_per_s": round(cg_rate, 2),
+        "toeplitz_build_s": round(toep_build_s, 4),
+        "recon": recon,
+        "kernels_ms_per_step": {k: round(v["total_ms"] / args.steps, 4) for k, v in kernels.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

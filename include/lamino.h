@@ -153,6 +153,21 @@ lam_status lam_nagopt(lam_ctx *ctx, const int64_t dims[3], lam_op_fn grad, lam_l
 lam_status lam_lipschitz(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void *user, int64_t max_iters,
                          double tol, uint64_t seed, double *L);
 
+/* Operator adapter over a Toeplitz operator for the solvers above:
+ * user = lam_toep*, y = sum_g sysmat_g(x).  (No counterpart in the reference,
+ * whose operators are std::function closures: src/mbir_cg.cpp:96-105.) */
+lam_status lam_toep_op(void *user, const float *const x[3], float *const y[3]);
+
+/* ---------------------------------------------------------- instrumentation */
+/* HIP-event timing of the library's own kernels on the context stream.
+ * Enabling clears earlier records.  (No reference counterpart; the reference
+ * only has a wall-clock Timer, include/timer.h:31-64.) */
+lam_status lam_timing_enable(lam_ctx *ctx, int on);
+/* Total device milliseconds and launch count recorded under a kernel name:
+ * "interp", "spread", "bin_points", "fft_axis", "deapod_pad", "crop_deapod",
+ * "toep_build", "toep_mul". */
+lam_status lam_timing_query(lam_ctx *ctx, const char *name, double *total_ms, int64_t *count);
+
 /* ---------------------------------------------------------- reconstruction jobs */
 typedef struct {
     int64_t nproj, nrows, ncols;  /* projection stack dims */

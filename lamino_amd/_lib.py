@@ -83,6 +83,9 @@ _SIGS = {
     "lam_nagopt": (ctypes.c_int, [c_vp, c_i64p, OP_FN, LOSS_FN, c_vp, ctypes.POINTER(c_vp), c_i64, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, c_i64]),
     "lam_lipschitz": (ctypes.c_int, [c_vp, c_i64p, OP_FN, c_vp, c_i64, ctypes.c_double, ctypes.c_uint64, c_dp]),
+    "lam_toep_op": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "lam_timing_enable": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "lam_timing_query": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_dp, c_i64p]),
     "lam_mbir": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
                                 ctypes.POINTER(lam_recon_params), c_vp, ctypes.POINTER(lam_recon_stats)]),
 }

@@ -618,6 +618,44 @@ lam_status lam_lipschitz(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void 
     });
 }
 
+lam_status lam_toep_op(void *user, const float *const x[3], float *const y[3]) {
+    return guard([&] {
+        need(user, "user"); need(x, "x"); need(y, "y");
+        toep_apply(*static_cast<lam_toep *>(user)->impl, CVol3{{x[0], x[1], x[2]}}, Vol3{{y[0], y[1], y[2]}}, 0.f);
+    });
+}
+
+lam_status lam_timing_enable(lam_ctx *ctx, int on) {
+    return guard([&] {
+        need(ctx, "ctx");
+        Ctx &c = ctx->impl;
+        LAM_HIP(hipStreamSynchronize(c.stream));
+        c.timer.clear();
+        c.timer.on = on != 0;
+    });
+}
+
+lam_status lam_timing_query(lam_ctx *ctx, const char *name, double *total_ms, int64_t *count) {
+    return guard([&] {
+        need(ctx, "ctx"); need(name, "name");
+        Ctx &c = ctx->impl;
+        LAM_HIP(hipStreamSynchronize(c.stream));
+        double tot = 0;
+        int64_t n = 0;
+        auto it = c.timer.ev.find(name);
+        if (it != c.timer.ev.end()) {
+            for (auto &pr : it->second) {
+                float ms = 0.f;
+                LAM_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+                tot += ms;
+                ++n;
+            }
+        }
+        if (total_ms) *total_ms = tot;
+        if (count) *count = n;
+    });
+}
+
 lam_status lam_mbir(lam_ctx *ctx, const lam_dataset *datasets, int nds, const int64_t recon_dims[3],
                     const lam_recon_params *params, float *out3, lam_recon_stats *stats) {
     return guard([&] {

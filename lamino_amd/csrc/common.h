@@ -71,8 +71,31 @@ struct FftPlan;     // fft.hip
 struct NufftPlan;   // nufft.hip
 struct Ctx;
 
+// HIP-event timing of named kernels on the context stream (off by default).
+struct KernelTimer {
+    bool on = false;
+    std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> ev;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t take() {
+        hipEvent_t e;
+        if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+        else LAM_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void clear() {
+        for (auto &kv : ev)
+            for (auto &pr : kv.second) { pool.push_back(pr.first); pool.push_back(pr.second); }
+        ev.clear();
+    }
+    ~KernelTimer() {
+        clear();
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 // ---------------------------------------------------------------- context
 struct Ctx {
+    KernelTimer timer;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -87,6 +110,24 @@ struct Ctx {
 };
 
 FftPlan &get_fft_plan(Ctx &cx, int n, int sign);
+
+// Records start/stop events around the launches made in its scope when the
+// context's timer is on.
+struct TimedScope {
+    Ctx &cx;
+    const char *name;
+    hipEvent_t stop = nullptr;
+    TimedScope(Ctx &c, const char *n) : cx(c), name(n) {
+        if (!cx.timer.on) return;
+        hipEvent_t start = cx.timer.take();
+        stop = cx.timer.take();
+        LAM_HIP(hipEventRecord(start, cx.stream));
+        cx.timer.ev[name].push_back({start, stop});
+    }
+    ~TimedScope() {
+        if (stop) (void)hipEventRecord(stop, cx.stream);
+    }
+};
 
 // Number of blocks for a 1-D grid-stride launch over n items.
 inline unsigned grid_for(size_t n, unsigned block = 256, unsigned cap = 8192) {

@@ -332,6 +332,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
     P.L = L;
     size_t lds = 2 * (size_t)L * plan.m * sizeof(float2);
+    TimedScope ts(cx, "fft_axis");
     if (inner)
         hipLaunchKernelGGL(k_fft_axis<1>, grid, dim3(kThreads), lds, cx.stream, P);
     else

@@ -25,14 +25,22 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     return i;
 }
 
-// ES kernel weights for one axis: cells i0 .. i0+W-1 around fine coordinate t.
-template <int W>
-__device__ __forceinline__ void es_weights(double t, int nf, float beta, int &i0, float *wts) {
+// Coordinate arithmetic is kept free of FMA contraction so that every kernel
+// that places a point (binning, spreading, interpolation) computes bitwise the
+// same fine coordinate and first kernel cell.
+
+// Folds t into [0, nf) and returns the first kernel cell ceil(t - W/2).
+__device__ __forceinline__ int first_cell(double &t, int nf, int W) {
+#pragma clang fp contract(off)
     const double fl = floor(t / (double)nf);
     t -= fl * (double)nf;
-    const double s = ceil(t - 0.5 * W);
-    i0 = (int)s;
-    const float z0 = (float)((s - t) * (2.0 / W));
+    return (int)ceil(t - 0.5 * W);
+}
+
+// ES kernel weights for one axis: cells i0 .. i0+W-1 around the folded coordinate t.
+template <int W>
+__device__ __forceinline__ void es_weights(double t, int i0, float beta, float *wts) {
+    const float z0 = (float)(((double)i0 - t) * (2.0 / W));
     const float dz = 2.0f / (float)W;
 #pragma unroll
     for (int m = 0; m < W; ++m) {
@@ -48,12 +56,43 @@ struct PointFrame {
 
 __device__ __forceinline__ void point_coords(const Geo &g, const PointFrame &f, int j, int k, double &tx, double &ty,
                                              double &tz) {
+#pragma clang fp contract(off)
     const double qX = ((double)k - f.k0) * f.dX;
     const double qY = ((double)j - f.j0) * f.dY;
     tx = (qX * g.ux + qY * g.vx) * f.sx;
     ty = (qY * g.vy) * f.sy;
     tz = (qX * g.uz + qY * g.vz) * f.sz;
 }
+
+// Placement of one point: folded coordinates, first cells, weights.
+template <int W>
+struct Placed {
+    int ix, iy, iz;
+    float wx[W], wy[W], wz[W];
+};
+
+template <int W>
+__device__ __forceinline__ void place(const Geo &g, const PointFrame &fr, int j, int k, int nfz, int nfy, int nfx,
+                                      float beta, Placed<W> &pt) {
+    double tx, ty, tz;
+    point_coords(g, fr, j, k, tx, ty, tz);
+    pt.ix = first_cell(tx, nfx, W);
+    pt.iy = first_cell(ty, nfy, W);
+    pt.iz = first_cell(tz, nfz, W);
+    es_weights<W>(tx, pt.ix, beta, pt.wx);
+    es_weights<W>(ty, pt.iy, beta, pt.wy);
+    es_weights<W>(tz, pt.iz, beta, pt.wz);
+}
+
+// Tiling of the fine grid for the spreader: tile (tz,ty,tx) owns the points
+// whose first kernel cell (wrapped into [0,nf)) lies in it, and accumulates
+// them over a (T+W-1)^3 halo box in LDS.
+struct BinParams {
+    int T[3];    // tile extent z, y, x
+    int nt[3];   // tiles per axis
+};
+
+__device__ __forceinline__ int wrap0(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ------------------------------------------------------------------ deapodise + pad
 // fine[g][c] = vol_c[i] * corr, where array index i <-> mode i - n/2 <-> fine index mode mod nf.
@@ -127,15 +166,13 @@ __global__ __launch_bounds__(256) void k_crop_deapod(const float2 *__restrict__ 
     }
 }
 
-// ------------------------------------------------------------------ interpolation (type 2)
-// HALF: NC = 3 real-volume components combined with the beam direction,
-// s = sum_i d_i c_i(q), written to the point and conj(s) to its mirror.
-// !HALF: NC = 1, every point.
+// ------------------------------------------------------------------ point binning
+// tile of every point (by its first kernel cell), and its rank inside the tile
 template <int W, bool HALF>
-__global__ __launch_bounds__(256) void k_interp(const float2 *__restrict__ fine, float2 *__restrict__ c,
-                                                const Geo *__restrict__ geo, long long P, int R, int C, int nfz,
-                                                int nfy, int nfx, float beta, PointFrame fr) {
-    constexpr int NC = HALF ? 3 : 1;
+__global__ __launch_bounds__(256) void k_bin_count(const Geo *__restrict__ geo, long long P, int R, int C, int nfz,
+                                                   int nfy, int nfx, PointFrame fr, BinParams bp,
+                                                   unsigned *__restrict__ tileid, unsigned *__restrict__ rank,
+                                                   unsigned *__restrict__ count) {
     const long long S = (long long)R * C;
     const long long H = HALF ? (S + 1) / 2 : S;
     const long long total = P * H;
@@ -144,43 +181,285 @@ __global__ __launch_bounds__(256) void k_interp(const float2 *__restrict__ fine,
         const long long p = gid / H;
         const long long idx = gid - p * H;
         const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
-        const Geo g = geo[p];
         double tx, ty, tz;
-        point_coords(g, fr, j, k, tx, ty, tz);
-        int ix, iy, iz;
-        float wx[W], wy[W], wz[W];
-        es_weights<W>(tx, nfx, beta, ix, wx);
-        es_weights<W>(ty, nfy, beta, iy, wy);
-        es_weights<W>(tz, nfz, beta, iz, wz);
+        point_coords(geo[p], fr, j, k, tx, ty, tz);
+        const int ix = wrap0(first_cell(tx, nfx, W), nfx);
+        const int iy = wrap0(first_cell(ty, nfy, W), nfy);
+        const int iz = wrap0(first_cell(tz, nfz, W), nfz);
+        const unsigned tile =
+            (unsigned)(((iz / bp.T[0]) * bp.nt[1] + (iy / bp.T[1])) * bp.nt[2] + (ix / bp.T[2]));
+        tileid[gid] = tile;
+        rank[gid] = atomicAdd(&count[tile], 1u);
+    }
+}
+
+// exclusive prefix sum of n counts, one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan_excl(const unsigned *__restrict__ count, unsigned *__restrict__ offset,
+                                                    long long n) {
+    __shared__ unsigned long long part[1024];
+    const long long per = (n + 1023) / 1024;
+    const long long b = threadIdx.x * per, e = b + per < n ? b + per : n;
+    unsigned long long s = 0;
+    for (long long i = b; i < e; ++i) s += count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        unsigned long long v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
+    for (long long i = b; i < e; ++i) {
+        offset[i] = (unsigned)run;
+        run += count[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const unsigned *__restrict__ tileid,
+                                                     const unsigned *__restrict__ rank,
+                                                     const unsigned *__restrict__ offset, unsigned *__restrict__ ids,
+                                                     long long total) {
+    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * blockDim.x)
+        ids[offset[tileid[gid]] + rank[gid]] = (unsigned)gid;
+}
+
+// ------------------------------------------------------------------ tiled kernels
+// A workgroup handles one chunk of the points binned to a tile (tz,ty,tx).
+// The tile's halo box covers the fine cells [o, o + T + W - 1) per axis, so
+// every kernel cell of its points lies inside it.
+struct TileBox {
+    int oz, oy, ox, HZ, HY, HX, ncell;
+};
+
+template <int W>
+__device__ __forceinline__ TileBox tile_box(int tile, const BinParams &bp) {
+    TileBox b;
+    const int tx = tile % bp.nt[2], ty = (tile / bp.nt[2]) % bp.nt[1], tz = tile / (bp.nt[2] * bp.nt[1]);
+    b.oz = tz * bp.T[0]; b.oy = ty * bp.T[1]; b.ox = tx * bp.T[2];
+    b.HZ = bp.T[0] + W - 1; b.HY = bp.T[1] + W - 1; b.HX = bp.T[2] + W - 1;
+    b.ncell = b.HZ * b.HY * b.HX;
+    return b;
+}
+
+__device__ __forceinline__ long long box_to_fine(const TileBox &b, int cell, int nfz, int nfy, int nfx) {
+    const int cx_ = cell % b.HX, cy_ = (cell / b.HX) % b.HY, cz_ = cell / (b.HX * b.HY);
+    const int gz = (b.oz + cz_) % nfz, gy = (b.oy + cy_) % nfy, gx = (b.ox + cx_) % nfx;
+    return ((long long)gz * nfy + gy) * nfx + gx;
+}
+
+// ---- spreading (type 1)
+// The box accumulator lives in LDS as NC complex planes.  The four waves of
+// the workgroup own the four (z, y) parity classes of box cells; every wave
+// walks all staged points and adds the contributions to its own cells with
+// plain read-modify-writes, so no cell is ever touched by two lanes or two
+// waves at once and no atomics are needed.  The box is then flushed once to
+// the fine grid with global atomics (neighbouring boxes overlap).
+constexpr int kStage = 128;   // points staged per batch
+
+template <int W, int NC>
+constexpr int stage_words() { return 3 + 3 * W + 2 * NC; }
+
+template <int W, bool HALF>
+__global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine, const float2 *__restrict__ c,
+                                                      const Geo *__restrict__ geo, long long P, int R, int C,
+                                                      int nfz, int nfy, int nfx, float beta, PointFrame fr,
+                                                      BinParams bp, const unsigned *__restrict__ ids,
+                                                      const uint4 *__restrict__ chunks) {
+    constexpr int NC = HALF ? 3 : 1;
+    constexpr int SW = stage_words<W, NC>();
+    constexpr int HW = W / 2;
+    constexpr int NOWN = HW * HW * W;  // cells of one point owned by one wave
+    extern __shared__ float lds[];
+    const uint4 ch = chunks[blockIdx.x];
+    const TileBox bx = tile_box<W>((int)ch.x, bp);
+    float2 *acc = reinterpret_cast<float2 *>(lds);          // [NC][ncell]
+    float *stg = lds + (size_t)2 * NC * bx.ncell;            // [kStage][SW]
+    for (int i = threadIdx.x; i < NC * bx.ncell; i += blockDim.x) acc[i] = make_float2(0.f, 0.f);
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int zpar = wave >> 1, ypar = wave & 1;
+    const long long S = (long long)R * C;
+    const long long H = HALF ? (S + 1) / 2 : S;
+
+    for (unsigned base = 0; base < ch.z; base += kStage) {
+        const int nb = (int)min((unsigned)kStage, ch.z - base);
+        __syncthreads();  // the box is zeroed / the previous batch is consumed
+        if ((int)threadIdx.x < nb) {
+            const long long gid = ids[ch.y + base + threadIdx.x];
+            const long long p = gid / H;
+            const long long idx = gid - p * H;
+            const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
+            const Geo g = geo[p];
+            float2 v = c[p * S + idx];
+            float2 vc[NC];
+            if constexpr (HALF) {
+                const long long mir = S - 1 - idx;
+                if (mir != idx) v = cadd(v, cconj(c[p * S + mir]));
+                vc[0] = cscale(v, (float)g.d0);
+                vc[1] = cscale(v, (float)g.d1);
+                vc[2] = cscale(v, (float)g.d2);
+            } else {
+                vc[0] = v;
+            }
+            Placed<W> pt;
+            place<W>(g, fr, j, k, nfz, nfy, nfx, beta, pt);
+            int lz = wrap0(pt.iz, nfz) - bx.oz, ly = wrap0(pt.iy, nfy) - bx.oy, lx = wrap0(pt.ix, nfx) - bx.ox;
+            float *sp = stg + threadIdx.x * SW;
+            if (!(lz >= 0 && lz < bp.T[0] && ly >= 0 && ly < bp.T[1] && lx >= 0 && lx < bp.T[2])) {
+                // a point outside its tile (not expected): spread it straight to the fine grid
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    const int zz = wrapi(pt.iz + a, nfz);
+#pragma unroll
+                    for (int b = 0; b < W; ++b) {
+                        const int yy = wrapi(pt.iy + b, nfy);
+                        float2 *row = fine + ((long long)zz * nfy + yy) * (long long)nfx * NC;
+                        const float wzy = pt.wz[a] * pt.wy[b];
+#pragma unroll
+                        for (int e = 0; e < W; ++e) {
+                            float *dst = reinterpret_cast<float *>(row + (long long)wrapi(pt.ix + e, nfx) * NC);
+                            const float wgt = wzy * pt.wx[e];
+#pragma unroll
+                            for (int q = 0; q < NC; ++q) {
+                                atomicAdd(dst + 2 * q, vc[q].x * wgt);
+                                atomicAdd(dst + 2 * q + 1, vc[q].y * wgt);
+                            }
+                        }
+                    }
+                }
+                lz = -1;
+            }
+            sp[0] = __int_as_float(lz);
+            sp[1] = __int_as_float(ly);
+            sp[2] = __int_as_float(lx);
+#pragma unroll
+            for (int m = 0; m < W; ++m) {
+                sp[3 + m] = pt.wz[m];
+                sp[3 + W + m] = pt.wy[m];
+                sp[3 + 2 * W + m] = pt.wx[m];
+            }
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                sp[3 + 3 * W + 2 * q] = vc[q].x;
+                sp[3 + 3 * W + 2 * q + 1] = vc[q].y;
+            }
+        }
+        __syncthreads();
+        for (int kk = 0; kk < nb; ++kk) {
+            const float *sp = stg + kk * SW;
+            const int lz = __float_as_int(sp[0]);
+            if (lz < 0) continue;
+            const int ly = __float_as_int(sp[1]), lx = __float_as_int(sp[2]);
+            const int zoff = (zpar - lz) & 1, yoff = (ypar - ly) & 1;
+            float2 vq[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) vq[q] = make_float2(sp[3 + 3 * W + 2 * q], sp[3 + 3 * W + 2 * q + 1]);
+            for (int cidx = lane; cidx < NOWN; cidx += 64) {
+                const int e = cidx % W, b = yoff + 2 * ((cidx / W) % HW), a = zoff + 2 * (cidx / (W * HW));
+                const float wgt = sp[3 + a] * sp[3 + W + b] * sp[3 + 2 * W + e];
+                const int cell = ((lz + a) * bx.HY + (ly + b)) * bx.HX + (lx + e);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    float2 t = acc[q * bx.ncell + cell];
+                    t.x += vq[q].x * wgt;
+                    t.y += vq[q].y * wgt;
+                    acc[q * bx.ncell + cell] = t;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int cell = threadIdx.x; cell < bx.ncell; cell += blockDim.x) {
+        float *dst = reinterpret_cast<float *>(fine + box_to_fine(bx, cell, nfz, nfy, nfx) * NC);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const float2 t = acc[q * bx.ncell + cell];
+            if (t.x != 0.f) atomicAdd(dst + 2 * q, t.x);
+            if (t.y != 0.f) atomicAdd(dst + 2 * q + 1, t.y);
+        }
+    }
+}
+
+// ---- interpolation (type 2), tiled: the tile's halo box of the fine grid is
+// staged in LDS once and every point of the chunk gathers from it.
+template <int W, bool HALF>
+__global__ __launch_bounds__(256) void k_interp_tiled(const float2 *__restrict__ fine, float2 *__restrict__ c,
+                                                      const Geo *__restrict__ geo, long long P, int R, int C,
+                                                      int nfz, int nfy, int nfx, float beta, PointFrame fr,
+                                                      BinParams bp, const unsigned *__restrict__ ids,
+                                                      const uint4 *__restrict__ chunks) {
+    constexpr int NC = HALF ? 3 : 1;
+    extern __shared__ float lds[];
+    const uint4 ch = chunks[blockIdx.x];
+    const TileBox bx = tile_box<W>((int)ch.x, bp);
+    float2 *box = reinterpret_cast<float2 *>(lds);   // [ncell][NC]
+    for (int i = threadIdx.x; i < NC * bx.ncell; i += blockDim.x) {
+        const int cell = i / NC, q = i - cell * NC;
+        box[i] = fine[box_to_fine(bx, cell, nfz, nfy, nfx) * NC + q];
+    }
+    __syncthreads();
+    const long long S = (long long)R * C;
+    const long long H = HALF ? (S + 1) / 2 : S;
+    for (unsigned i = threadIdx.x; i < ch.z; i += blockDim.x) {
+        const long long gid = ids[ch.y + i];
+        const long long p = gid / H;
+        const long long idx = gid - p * H;
+        const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
+        const Geo g = geo[p];
+        Placed<W> pt;
+        place<W>(g, fr, j, k, nfz, nfy, nfx, beta, pt);
+        const int lz = wrap0(pt.iz, nfz) - bx.oz, ly = wrap0(pt.iy, nfy) - bx.oy, lx = wrap0(pt.ix, nfx) - bx.ox;
         float2 acc[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[q] = make_float2(0.f, 0.f);
-        int xs[W];
+        if (lz >= 0 && lz < bp.T[0] && ly >= 0 && ly < bp.T[1] && lx >= 0 && lx < bp.T[2]) {
 #pragma unroll
-        for (int e = 0; e < W; ++e) xs[e] = wrapi(ix + e, nfx);
-        for (int a = 0; a < W; ++a) {
-            const int zz = wrapi(iz + a, nfz);
+            for (int a = 0; a < W; ++a) {
 #pragma unroll
-            for (int b = 0; b < W; ++b) {
-                const int yy = wrapi(iy + b, nfy);
-                const float2 *row = fine + ((long long)zz * nfy + yy) * (long long)nfx * NC;
-                float2 racc[NC];
+                for (int b = 0; b < W; ++b) {
+                    const float2 *row = box + (size_t)(((lz + a) * bx.HY + (ly + b)) * bx.HX + lx) * NC;
+                    float2 racc[NC];
 #pragma unroll
-                for (int q = 0; q < NC; ++q) racc[q] = make_float2(0.f, 0.f);
+                    for (int q = 0; q < NC; ++q) racc[q] = make_float2(0.f, 0.f);
 #pragma unroll
-                for (int e = 0; e < W; ++e) {
+                    for (int e = 0; e < W; ++e) {
+#pragma unroll
+                        for (int q = 0; q < NC; ++q) {
+                            const float2 v = row[e * NC + q];
+                            racc[q].x += v.x * pt.wx[e];
+                            racc[q].y += v.y * pt.wx[e];
+                        }
+                    }
+                    const float wzy = pt.wz[a] * pt.wy[b];
 #pragma unroll
                     for (int q = 0; q < NC; ++q) {
-                        const float2 v = row[(long long)xs[e] * NC + q];
-                        racc[q].x += v.x * wx[e];
-                        racc[q].y += v.y * wx[e];
+                        acc[q].x += racc[q].x * wzy;
+                        acc[q].y += racc[q].y * wzy;
                     }
                 }
-                const float wzy = wz[a] * wy[b];
+            }
+        } else {
+            // a point outside its tile (not expected): gather from the fine grid
 #pragma unroll
-                for (int q = 0; q < NC; ++q) {
-                    acc[q].x += racc[q].x * wzy;
-                    acc[q].y += racc[q].y * wzy;
+            for (int a = 0; a < W; ++a) {
+                const int zz = wrapi(pt.iz + a, nfz);
+#pragma unroll
+                for (int b = 0; b < W; ++b) {
+                    const int yy = wrapi(pt.iy + b, nfy);
+                    const float2 *row = fine + ((long long)zz * nfy + yy) * (long long)nfx * NC;
+                    const float wzy = pt.wz[a] * pt.wy[b];
+#pragma unroll
+                    for (int e = 0; e < W; ++e) {
+                        const float wgt = wzy * pt.wx[e];
+#pragma unroll
+                        for (int q = 0; q < NC; ++q) {
+                            const float2 v = row[(long long)wrapi(pt.ix + e, nfx) * NC + q];
+                            acc[q].x += v.x * wgt;
+                            acc[q].y += v.y * wgt;
+                        }
+                    }
                 }
             }
         }
@@ -194,63 +473,6 @@ __global__ __launch_bounds__(256) void k_interp(const float2 *__restrict__ fine,
             if (mir != idx) c[p * S + mir] = cconj(s);
         } else {
             c[p * S + idx] = acc[0];
-        }
-    }
-}
-
-// ------------------------------------------------------------------ spreading (type 1)
-template <int W, bool HALF>
-__global__ __launch_bounds__(256) void k_spread_atomic(float2 *__restrict__ fine, const float2 *__restrict__ c,
-                                                       const Geo *__restrict__ geo, long long P, int R, int C,
-                                                       int nfz, int nfy, int nfx, float beta, PointFrame fr) {
-    constexpr int NC = HALF ? 3 : 1;
-    const long long S = (long long)R * C;
-    const long long H = HALF ? (S + 1) / 2 : S;
-    const long long total = P * H;
-    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
-         gid += (long long)gridDim.x * blockDim.x) {
-        const long long p = gid / H;
-        const long long idx = gid - p * H;
-        const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
-        const Geo g = geo[p];
-        float2 v = c[p * S + idx];
-        if (HALF) {
-            const long long mir = S - 1 - idx;
-            if (mir != idx) v = cadd(v, cconj(c[p * S + mir]));
-        }
-        float2 vc[NC];
-        if constexpr (HALF) {
-            vc[0] = cscale(v, (float)g.d0);
-            vc[1] = cscale(v, (float)g.d1);
-            vc[2] = cscale(v, (float)g.d2);
-        } else {
-            vc[0] = v;
-        }
-        double tx, ty, tz;
-        point_coords(g, fr, j, k, tx, ty, tz);
-        int ix, iy, iz;
-        float wx[W], wy[W], wz[W];
-        es_weights<W>(tx, nfx, beta, ix, wx);
-        es_weights<W>(ty, nfy, beta, iy, wy);
-        es_weights<W>(tz, nfz, beta, iz, wz);
-        for (int a = 0; a < W; ++a) {
-            const int zz = wrapi(iz + a, nfz);
-            for (int b = 0; b < W; ++b) {
-                const int yy = wrapi(iy + b, nfy);
-                float2 *row = fine + ((long long)zz * nfy + yy) * (long long)nfx * NC;
-                const float wzy = wz[a] * wy[b];
-#pragma unroll
-                for (int e = 0; e < W; ++e) {
-                    const int xx = wrapi(ix + e, nfx);
-                    const float wgt = wzy * wx[e];
-#pragma unroll
-                    for (int q = 0; q < NC; ++q) {
-                        float *dst = reinterpret_cast<float *>(row + (long long)xx * NC + q);
-                        atomicAdd(dst, vc[q].x * wgt);
-                        atomicAdd(dst + 1, vc[q].y * wgt);
-                    }
-                }
-            }
         }
     }
 }
@@ -308,40 +530,134 @@ PointFrame frame(const Grid &g, const NufftPlan &pl) {
         default: throw Error(LAM_ERR_NUFFT, "Error in finufft_makeplan: unsupported kernel width"); \
     }
 
-void launch_interp(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, bool half) {
+constexpr unsigned kChunk = 2048;  // points per spreading workgroup
+
+constexpr size_t kLdsBudget = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
+
+size_t spread_lds_bytes(int T, int w, int nc) {
+    const size_t h = (size_t)(T + w - 1);
+    return h * h * h * nc * 8 + (size_t)kStage * (3 + 3 * w + 2 * nc) * 4;
+}
+
+BinParams bin_params(const NufftPlan &pl, int nc) {
+    BinParams bp;
+    int T = 2;
+    for (int cand : {16, 12, 10, 8, 6, 4, 2}) {
+        if (spread_lds_bytes(cand, pl.w, nc) <= kLdsBudget) { T = cand; break; }
+    }
+    for (int a = 0; a < 3; ++a) {
+        bp.T[a] = T;
+        bp.nt[a] = (pl.nf[a] + T - 1) / T;
+    }
+    return bp;
+}
+
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024) LAM_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinParams &bp) {
+    std::vector<long long> key = {pl.nf[0], pl.nf[1], pl.nf[2], pl.w, half ? 1 : 0, bp.T[0], bp.T[1], bp.T[2]};
+    auto it = g.bins.find(key);
+    if (it != g.bins.end()) return *it->second;
+    TimedScope ts(cx, "bin_points");
+    const long long S = g.R * g.C, H = half ? (S + 1) / 2 : S, total = g.P * H;
+    LAM_REQUIRE(total < (1ll << 32), LAM_ERR_ARG, "too many points in one polar grid for 32-bit bins");
+    const long long ntiles = (long long)bp.nt[0] * bp.nt[1] * bp.nt[2];
+    DevBuf tileid(sizeof(unsigned) * total), rank(sizeof(unsigned) * total);
+    DevBuf count(sizeof(unsigned) * ntiles), offset(sizeof(unsigned) * ntiles);
+    LAM_HIP(hipMemsetAsync(count.ptr, 0, count.bytes, cx.stream));
     const PointFrame fr = frame(g, pl);
-    const long long S = g.R * g.C;
-    const long long total = g.P * (half ? (S + 1) / 2 : S);
     const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
     const Geo *geo = g.geo.as<Geo>();
     if (half) {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_interp<W, true>), dim3(grid), dim3(256), 0, cx.stream,
-                                                  pl.fine.as<float2>(), c, geo, g.P, (int)g.R, (int)g.C, pl.nf[0],
-                                                  pl.nf[1], pl.nf[2], pl.beta, fr));
+        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_count<W, true>), dim3(grid), dim3(256), 0, cx.stream, geo, g.P,
+                                                  (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
+                                                  tileid.as<unsigned>(), rank.as<unsigned>(), count.as<unsigned>()));
     } else {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_interp<W, false>), dim3(grid), dim3(256), 0, cx.stream,
-                                                  pl.fine.as<float2>(), c, geo, g.P, (int)g.R, (int)g.C, pl.nf[0],
-                                                  pl.nf[1], pl.nf[2], pl.beta, fr));
+        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_count<W, false>), dim3(grid), dim3(256), 0, cx.stream, geo,
+                                                  g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
+                                                  tileid.as<unsigned>(), rank.as<unsigned>(), count.as<unsigned>()));
+    }
+    check_launch();
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, cx.stream, count.as<unsigned>(), offset.as<unsigned>(),
+                       ntiles);
+    check_launch();
+    auto sp = std::make_unique<SortedPoints>();
+    sp->ids.alloc(sizeof(unsigned) * total);
+    sp->npoints = total;
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, cx.stream, tileid.as<unsigned>(), rank.as<unsigned>(),
+                       offset.as<unsigned>(), sp->ids.as<unsigned>(), total);
+    check_launch();
+    std::vector<unsigned> hc(ntiles);
+    LAM_HIP(hipMemcpyAsync(hc.data(), count.ptr, sizeof(unsigned) * ntiles, hipMemcpyDeviceToHost, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+    std::vector<uint4> chunks;
+    unsigned long long off = 0;
+    for (long long t = 0; t < ntiles; ++t) {
+        for (unsigned s0 = 0; s0 < hc[t]; s0 += kChunk)
+            chunks.push_back(make_uint4((unsigned)t, (unsigned)(off + s0), std::min(kChunk, hc[t] - s0), 0u));
+        off += hc[t];
+    }
+    sp->nchunks = (long long)chunks.size();
+    if (!chunks.empty()) {
+        sp->chunks.alloc(sizeof(uint4) * chunks.size());
+        LAM_HIP(hipMemcpyAsync(sp->chunks.ptr, chunks.data(), sizeof(uint4) * chunks.size(), hipMemcpyHostToDevice,
+                               cx.stream));
+        LAM_HIP(hipStreamSynchronize(cx.stream));
+    }
+    SortedPoints &ref = *sp;
+    g.bins.emplace(std::move(key), std::move(sp));
+    return ref;
+}
+
+template <int W, bool HALF>
+void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams &bp, SortedPoints &sp) {
+    constexpr int NC = HALF ? 3 : 1;
+    const size_t lds = spread_lds_bytes(bp.T[0], W, NC);
+    allow_lds(&k_spread_tiled<W, HALF>, lds);
+    hipLaunchKernelGGL((k_spread_tiled<W, HALF>), dim3((unsigned)sp.nchunks), dim3(256), lds, cx.stream,
+                       pl.fine.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2],
+                       pl.beta, frame(g, pl), bp, sp.ids.as<unsigned>(), sp.chunks.as<uint4>());
+}
+
+template <int W, bool HALF>
+void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, SortedPoints &sp) {
+    constexpr int NC = HALF ? 3 : 1;
+    const size_t h = (size_t)(bp.T[0] + W - 1);
+    const size_t lds = h * h * h * NC * 8;
+    allow_lds(&k_interp_tiled<W, HALF>, lds);
+    hipLaunchKernelGGL((k_interp_tiled<W, HALF>), dim3((unsigned)sp.nchunks), dim3(256), lds, cx.stream,
+                       pl.fine.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2],
+                       pl.beta, frame(g, pl), bp, sp.ids.as<unsigned>(), sp.chunks.as<uint4>());
+}
+
+void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) {
+    const int nc = half ? 3 : 1;
+    const BinParams bp = bin_params(pl, nc);
+    SortedPoints &sp = get_bins(cx, g, pl, half, bp);
+    LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc, cx.stream));
+    if (sp.nchunks == 0) return;
+    TimedScope ts(cx, "spread");
+    if (half) {
+        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, true>(cx, g, pl, c, bp, sp)));
+    } else {
+        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, false>(cx, g, pl, c, bp, sp)));
     }
     check_launch();
 }
 
-void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) {
-    const PointFrame fr = frame(g, pl);
-    const long long S = g.R * g.C;
-    const long long total = g.P * (half ? (S + 1) / 2 : S);
-    const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
-    const Geo *geo = g.geo.as<Geo>();
+void launch_interp(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, bool half) {
     const int nc = half ? 3 : 1;
-    LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc, cx.stream));
+    const BinParams bp = bin_params(pl, nc);
+    SortedPoints &sp = get_bins(cx, g, pl, half, bp);
+    if (sp.nchunks == 0) return;
+    TimedScope ts(cx, "interp");
     if (half) {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_spread_atomic<W, true>), dim3(grid), dim3(256), 0, cx.stream,
-                                                  pl.fine.as<float2>(), c, geo, g.P, (int)g.R, (int)g.C, pl.nf[0],
-                                                  pl.nf[1], pl.nf[2], pl.beta, fr));
+        LAM_WIDTH_SWITCH(pl.w, (interp_w<W, true>(cx, g, pl, c, bp, sp)));
     } else {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_spread_atomic<W, false>), dim3(grid), dim3(256), 0, cx.stream,
-                                                  pl.fine.as<float2>(), c, geo, g.P, (int)g.R, (int)g.C, pl.nf[0],
-                                                  pl.nf[1], pl.nf[2], pl.beta, fr));
+        LAM_WIDTH_SWITCH(pl.w, (interp_w<W, false>(cx, g, pl, c, bp, sp)));
     }
     check_launch();
 }
@@ -351,6 +667,7 @@ void ensure_fine(NufftPlan &pl, int nc) {
 }
 
 void deapod_pad(Ctx &cx, NufftPlan &pl, const float *const m[3], const float2 *cin, int nc) {
+    TimedScope ts(cx, "deapod_pad");
     const long long total = (long long)pl.nf[0] * pl.nf[1] * pl.nf[2];
     const unsigned grid = grid_for((size_t)total, 256, 1u << 16);
     if (nc == 3)
@@ -366,6 +683,7 @@ void deapod_pad(Ctx &cx, NufftPlan &pl, const float *const m[3], const float2 *c
 }
 
 void crop_deapod(Ctx &cx, NufftPlan &pl, float *const out[3], float2 *cout, int nc, float scale, bool accumulate) {
+    TimedScope ts(cx, "crop_deapod");
     const long long total = pl.n[0] * pl.n[1] * pl.n[2];
     const unsigned grid = grid_for((size_t)total, 256, 1u << 16);
     if (nc == 3)

@@ -33,9 +33,12 @@ __global__ __launch_bounds__(256) void k_reduce(RedSpec s, double *partials) {
             }
         }
     }
-    for (int q = 0; q < s.nterms; ++q) {
-        const double v = block_sum_256(acc[q]);
-        if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = v;
+#pragma unroll
+    for (int q = 0; q < kMaxRedTerms; ++q) {
+        if (q < s.nterms) {
+            const double v = block_sum_256(acc[q]);
+            if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = v;
+        }
     }
 }
 

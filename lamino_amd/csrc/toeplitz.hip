@@ -126,9 +126,12 @@ __global__ __launch_bounds__(256) void k_toep_quad(const float2 *__restrict__ za
         }
     }
     const int nt = 3 * s.G;
-    for (int q = 0; q < nt; ++q) {
-        const double v = block_sum_256(acc[q]);
-        if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = v;
+#pragma unroll
+    for (int q = 0; q < kMaxQuadGrids * 3; ++q) {
+        if (q < nt) {
+            const double v = block_sum_256(acc[q]);
+            if (threadIdx.x == 0) partials[(size_t)q * gridDim.x + blockIdx.x] = v;
+        }
     }
 }
 
@@ -249,6 +252,7 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
             LAM_HIP(hipMemsetAsync(dst, 0, kbytes, cx.stream));
         }
         const double scale = 1.0 / ((double)gr.R * (double)gr.C * (double)Lt);
+        TimedScope ts(cx, "toep_build");
         hipLaunchKernelGGL(k_toep_build, dim3(lgrid(e)), dim3(256), 0, cx.stream, dst, gr.geo.as<Geo>(), (int)gr.P,
                            (int)gr.R, (int)gr.C, (int)dims[0], (int)dims[1], (int)dims[2], L1, L2, L3, scale);
         check_launch();
@@ -268,6 +272,7 @@ void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu) {
     Ctx &cx = *t.cx;
     forward_spectra(t, x);
     const long long Lt = (long long)t.L[0] * t.L[1] * t.L[2];
+    TimedScope ts(cx, "toep_mul");
     hipLaunchKernelGGL(k_toep_mul, dim3(lgrid(Lt)), dim3(256), 0, cx.stream, t.za.as<float2>(), t.zb.as<float2>(),
                        t.khat.as<float2>(), t.wa.as<float2>(), t.wb.as<float2>(), t.L[0], t.L[1], t.L[2]);
     check_launch();

@@ -66,6 +66,7 @@ template <class T> struct Fft1 {
     int sign = -1;
     std::vector<int> factors;           // radix sequence (smooth path)
     std::vector<cplx<T>> roots;         // exp(sign 2 pi i m / n), m < n
+    std::vector<std::vector<cplx<T>>> tw;  // per stage: exp(sign 2 pi i t k / (Ns r)), [t][k]
     bool bluestein = false;
     size_t m = 0;                       // Bluestein inner length (power of two)
     std::unique_ptr<Fft1<T>> inner_f, inner_b;
@@ -92,6 +93,14 @@ template <class T> struct Fft1 {
                 long double a = (long double)sign * 2.0L * (long double)kPi * (long double)i / (long double)n;
                 roots[i] = cplx<T>((T)std::cos(a), (T)std::sin(a));
             }
+            size_t Ns = 1;
+            for (int r : factors) {
+                std::vector<cplx<T>> t((size_t)r * Ns);
+                for (int q = 0; q < r; ++q)
+                    for (size_t k = 0; k < Ns; ++k) t[q * Ns + k] = roots[(q * k * (n / (Ns * r))) % n];
+                tw.push_back(std::move(t));
+                Ns *= r;
+            }
             return;
         }
         bluestein = true;
@@ -112,43 +121,76 @@ template <class T> struct Fft1 {
             if (i) b[m - i] = std::conj(chirp[i]);
         }
         bhat.resize(m);
-        inner_f->exec(b.data(), bhat.data());
+        std::vector<cplx<T>> work(m);
+        inner_f->exec(b.data(), bhat.data(), work.data());
         for (auto &v : bhat) v /= (T)m;
     }
 
-    void rec(const cplx<T> *in, cplx<T> *out, size_t len, size_t stride, size_t fi,
-             cplx<T> *scratch) const {
-        if (len == 1) { out[0] = in[0]; return; }
-        int p = factors[fi];
-        size_t sub = len / p;
-        for (int q = 0; q < p; ++q) rec(in + q * stride, out + q * sub, sub, stride * p, fi + 1, scratch);
-        size_t rs = n / len;  // root-table step for length len
-        cplx<T> t[16];
-        for (size_t k = 0; k < sub; ++k) {
-            for (int q = 0; q < p; ++q) t[q] = out[q * sub + k] * roots[(q * k * rs) % n];
-            for (int s = 0; s < p; ++s) {
-                cplx<T> acc(0);
-                for (int q = 0; q < p; ++q) acc += t[q] * roots[((size_t)(q * s) % p) * (n / p)];
-                scratch[s] = acc;
+    // One radix-r pass of the iterative (self-sorting) transform:
+    // y[(j/Ns) Ns r + k + t Ns] = sum_q W_r^{qt} (W_{Ns r}^{qk} x[j + q n/r]), k = j mod Ns.
+    void pass(const cplx<T> *x, cplx<T> *y, int r, size_t Ns, const cplx<T> *t) const {
+        const size_t step = n / r;
+        const T s = (T)sign;
+        for (size_t j = 0; j < step; ++j) {
+            const size_t k = j % Ns;
+            const size_t o = (j / Ns) * Ns * r + k;
+            cplx<T> v[16];
+            v[0] = x[j];
+            for (int q = 1; q < r; ++q) v[q] = x[j + q * step] * t[q * Ns + k];
+            if (r == 2) {
+                y[o] = v[0] + v[1];
+                y[o + Ns] = v[0] - v[1];
+            } else if (r == 4) {
+                const cplx<T> a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3];
+                const cplx<T> d = (v[1] - v[3]) * cplx<T>(0, s);
+                y[o] = a + c;
+                y[o + Ns] = b + d;
+                y[o + 2 * Ns] = a - c;
+                y[o + 3 * Ns] = b - d;
+            } else if (r == 3) {
+                const cplx<T> a = v[1] + v[2];
+                const cplx<T> b = v[0] - a * (T)0.5;
+                const cplx<T> c = (v[1] - v[2]) * cplx<T>(0, s * (T)0.86602540378443864676);
+                y[o] = v[0] + a;
+                y[o + Ns] = b + c;
+                y[o + 2 * Ns] = b - c;
+            } else {
+                for (int q = 0; q < r; ++q) {
+                    cplx<T> acc = v[0];
+                    for (int p = 1; p < r; ++p) acc += v[p] * roots[((size_t)(p * q) % r) * (n / r)];
+                    y[o + q * Ns] = acc;
+                }
             }
-            for (int s = 0; s < p; ++s) out[k + sub * s] = scratch[s];
         }
     }
 
-    // out-of-place transform of a contiguous length-n vector
-    void exec(const cplx<T> *in, cplx<T> *out) const {
+    // out-of-place transform of a contiguous length-n vector (work: n scratch, m for Bluestein)
+    void exec(const cplx<T> *in, cplx<T> *out, cplx<T> *work) const {
         if (n == 1) { out[0] = in[0]; return; }
         if (!bluestein) {
-            cplx<T> scratch[16];
-            rec(in, out, n, 1, 0, scratch);
+            const int S = (int)factors.size();
+            // ping-pong so that the last pass lands in out
+            cplx<T> *bufs[2] = {out, work};
+            int cur = (S % 2 == 0) ? 0 : 1;
+            std::copy(in, in + n, bufs[cur]);
+            size_t Ns = 1;
+            for (int st = 0; st < S; ++st) {
+                pass(bufs[cur], bufs[cur ^ 1], factors[st], Ns, tw[st].data());
+                cur ^= 1;
+                Ns *= factors[st];
+            }
             return;
         }
-        std::vector<cplx<T>> a(m, cplx<T>(0)), A(m);
+        std::vector<cplx<T>> a(m, cplx<T>(0)), A(m), w2(m);
         for (size_t i = 0; i < n; ++i) a[i] = in[i] * chirp[i];
-        inner_f->exec(a.data(), A.data());
+        inner_f->exec(a.data(), A.data(), w2.data());
         for (size_t i = 0; i < m; ++i) A[i] *= bhat[i];
-        inner_b->exec(A.data(), a.data());
+        inner_b->exec(A.data(), a.data(), w2.data());
         for (size_t i = 0; i < n; ++i) out[i] = a[i] * chirp[i];
+    }
+    void exec(const cplx<T> *in, cplx<T> *out) const {
+        std::vector<cplx<T>> w(std::max(n, m) + 1);
+        exec(in, out, w.data());
     }
 };
 
@@ -161,7 +203,7 @@ void fft_axis(cplx<T> *data, const Dims &d, int axis, int sign, bool parallel = 
     Fft1<T> plan(len, sign);
 #pragma omp parallel if (parallel)
     {
-        std::vector<cplx<T>> a(len), b(len);
+        std::vector<cplx<T>> a(len), b(len), w(std::max(len, plan.m) + 1);
 #pragma omp for schedule(static)
         for (long long line = 0; line < (long long)nlines; ++line) {
             size_t base;
@@ -169,7 +211,7 @@ void fft_axis(cplx<T> *data, const Dims &d, int axis, int sign, bool parallel = 
             else if (axis == 1) base = ((size_t)line / d.n3) * d.n2 * d.n3 + ((size_t)line % d.n3);
             else base = (size_t)line;
             for (size_t i = 0; i < len; ++i) a[i] = data[base + i * stride];
-            plan.exec(a.data(), b.data());
+            plan.exec(a.data(), b.data(), w.data());
             for (size_t i = 0; i < len; ++i) data[base + i * stride] = b[i];
         }
     }
