@@ -37,9 +37,13 @@ struct AxisPass {
     long long is1 = 0, is2 = 0, isi = 0, isn = 1;
     long long os1 = 0, os2 = 0, osi = 0, osn = 1;
     int c1 = 1, c2 = 1, ci = 1;
-    int nv = -1, ns = -1;      // -1: full length
+    int nv = -1, ns = -1;      // -1: full length; valid loads m < nv, stored outputs m < ns
+    int nv2 = 0, ns2 = 0;      // additionally valid / stored: the last nv2 / ns2 positions (wrapped ranges)
     int rot_in = 0, rot_out = 0;
     float scale = 1.0f;
+    // line sets with a hole: line o maps to o < split ? o : o + gap (wrapped mode ranges)
+    int o1_split = 1 << 30, o1_gap = 0;
+    int o2_split = 1 << 30, o2_gap = 0;
 };
 
 void run_axis(Ctx &cx, const FftPlan &plan, AxisPass p);

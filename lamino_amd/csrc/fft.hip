@@ -8,16 +8,19 @@ namespace lam {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kLdsComplex = 4096;  // per buffer: L * m <= 4096 complex -> 2 buffers = 64 KiB
+constexpr int kThreads = 512;
+constexpr int kLdsComplex = 4096;  // one LDS buffer per workgroup: L * m <= 4096 complex (32 KiB)
 
 struct KParams {
     AxisPass a;
-    int n, m, blue, nrad, L, sign;
+    int n, m, blue, nrad, sign;
+    int L, Li, Lo;  // lines per workgroup, L = Li * Lo
     int rad[24];
     const float2 *roots, *chirp, *bhat;
 };
 
+// LDS position of element n of tile line l: n-major when the tile runs along
+// the inner (contiguous) index, line-major otherwise.
 template <int INNER>
 __device__ __forceinline__ int lidx(int l, int n, int L, int M) {
     return INNER ? n * L + l : l * M + n;
@@ -69,85 +72,96 @@ __device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, in
         v[4] = make_float2(r1.x + i1.y, r1.y - i1.x);
         v[2] = make_float2(r2.x - i2.y, r2.y + i2.x);
         v[3] = make_float2(r2.x + i2.y, r2.y - i2.x);
-    } else {
-        // generic small prime: W_R^x = roots[(x mod R) * mstep] (mstep = m / R)
-        float2 o[R];
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            float2 acc = v[0];
-#pragma unroll
-            for (int t = 1; t < R; ++t) acc = cadd(acc, cmul(v[t], twiddle(roots, ((t * s) % R) * mstep, sign)));
-            o[s] = acc;
-        }
-#pragma unroll
-        for (int s = 0; s < R; ++s) v[s] = o[s];
     }
 }
 
 template <int R, int INNER>
-__device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__restrict__ dst, int L, int M,
-                                      int Ns, int sign, const float2 *roots) {
+__device__ __forceinline__ void stage(float2 *__restrict__ buf, int L, int M, int Ns, int sign,
+                                      const float2 *roots) {
+    constexpr int MAXB = (kLdsComplex / R + kThreads - 1) / kThreads;
     const int nbf = M / R;
+    const int tot = L * nbf;
     const int tstep = M / (Ns * R);
-    for (int e = threadIdx.x; e < L * nbf; e += blockDim.x) {
-        int l, j;
-        if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
-        const int k = j % Ns;
-        float2 v[R];
+    float2 v[MAXB][R];
 #pragma unroll
-        for (int t = 0; t < R; ++t) v[t] = src[lidx<INNER>(l, j + t * nbf, L, M)];
-        if (Ns > 1) {
+    for (int b = 0; b < MAXB; ++b) {
+        const int e = threadIdx.x + b * kThreads;
+        if (e < tot) {
+            int l, j;
+            if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], twiddle(roots, t * k * tstep, sign));
+            for (int t = 0; t < R; ++t) v[b][t] = buf[lidx<INNER>(l, j + t * nbf, L, M)];
         }
-        dft<R>(v, sign, roots, M / R);
-        const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int t = 0; t < R; ++t) dst[lidx<INNER>(l, base + t * Ns, L, M)] = v[t];
     }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int e = threadIdx.x + b * kThreads;
+        if (e < tot) {
+            int l, j;
+            if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
+            const int k = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], twiddle(roots, t * k * tstep, sign));
+            }
+            dft<R>(v[b], sign, roots, M / R);
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int t = 0; t < R; ++t) buf[lidx<INNER>(l, base + t * Ns, L, M)] = v[b][t];
+        }
+    }
+    __syncthreads();
 }
 
 template <int INNER>
-__device__ void stages(float2 *&src, float2 *&dst, const KParams &P, int sign) {
+__device__ void stages(float2 *buf, const KParams &P, int sign) {
     int Ns = 1;
     for (int s = 0; s < P.nrad; ++s) {
         const int r = P.rad[s];
         switch (r) {
-            case 2: stage<2, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            case 3: stage<3, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            case 4: stage<4, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            case 5: stage<5, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            case 7: stage<7, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            case 11: stage<11, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
-            default: stage<13, INNER>(src, dst, P.L, P.m, Ns, sign, P.roots); break;
+            case 2: stage<2, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
+            case 3: stage<3, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
+            case 4: stage<4, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
+            default: stage<5, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
         }
-        __syncthreads();
-        float2 *t = src; src = dst; dst = t;
         Ns *= r;
     }
 }
 
+__device__ __forceinline__ int remap(int o, int split, int gap) { return o < split ? o : o + gap; }
+
+// Global coordinates of tile line l; false when the line lies past the set.
+__device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, int ibase, long long &o1, int &o2,
+                                          int &i) {
+    const int lo = l / P.Li, li = l - lo * P.Li;
+    o2 = o2base + lo;
+    i = ibase + li;
+    if (o2 >= P.a.c2 || i >= P.a.ci) return false;
+    o1 = remap((int)blockIdx.z, P.a.o1_split, P.a.o1_gap);
+    o2 = remap(o2, P.a.o2_split, P.a.o2_gap);
+    return true;
+}
+
 template <int INNER>
 __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
-    extern __shared__ float2 lds[];
+    extern __shared__ float2 buf[];
     const int L = P.L, M = P.m, N = P.n;
-    float2 *bufA = lds, *bufB = lds + (size_t)L * M;
     const AxisPass &a = P.a;
-    const long long o1 = blockIdx.z;
-    const int o2base = INNER ? (int)blockIdx.y : (int)blockIdx.x * L;
-    const int ibase = INNER ? (int)blockIdx.x * L : (int)blockIdx.y;
+    const int o2base = INNER ? (int)blockIdx.y * P.Lo : (int)blockIdx.x * P.Lo;
+    const int ibase = INNER ? (int)blockIdx.x * P.Li : (int)blockIdx.y;
     const int nv = a.nv < 0 ? N : a.nv;
     const int ns = a.ns < 0 ? N : a.ns;
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
         int l, n;
         if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
-        const int o2 = INNER ? o2base : o2base + l;
-        const int i = INNER ? ibase + l : ibase;
+        long long o1;
+        int o2, i;
         int mm = n + a.rot_in;
         if (mm >= N) mm -= N;
         float2 v = make_float2(0.f, 0.f);
-        if (o2 < a.c2 && i < a.ci && mm < nv) {
+        if (tile_line(P, l, o2base, ibase, o1, o2, i) && (mm < nv || mm >= N - a.nv2)) {
             const long long off = o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi + (long long)mm * a.isn;
             if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
             else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
@@ -157,42 +171,41 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
             }
         }
         if (P.blue) v = cmul(v, P.chirp[n]);
-        bufA[lidx<INNER>(l, n, L, M)] = v;
+        buf[lidx<INNER>(l, n, L, M)] = v;
     }
     if (P.blue) {
         const int pad = M - N;
         for (int e = threadIdx.x; e < L * pad; e += blockDim.x) {
             int l, n;
             if (INNER) { l = e % L; n = N + e / L; } else { n = N + e % pad; l = e / pad; }
-            bufA[lidx<INNER>(l, n, L, M)] = make_float2(0.f, 0.f);
+            buf[lidx<INNER>(l, n, L, M)] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
 
-    float2 *src = bufA, *dst = bufB;
     if (!P.blue) {
-        stages<INNER>(src, dst, P, P.sign);
+        stages<INNER>(buf, P, P.sign);
     } else {
-        stages<INNER>(src, dst, P, -1);
+        stages<INNER>(buf, P, -1);
         for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
             int l, n;
             if (INNER) { l = e % L; n = e / L; } else { n = e % M; l = e / M; }
             const int id = lidx<INNER>(l, n, L, M);
-            src[id] = cmul(src[id], P.bhat[n]);
+            buf[id] = cmul(buf[id], P.bhat[n]);
         }
         __syncthreads();
-        stages<INNER>(src, dst, P, +1);
+        stages<INNER>(buf, P, +1);
     }
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
         int l, n;
         if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
-        const int o2 = INNER ? o2base : o2base + l;
-        const int i = INNER ? ibase + l : ibase;
+        long long o1;
+        int o2, i;
         int mm = n + a.rot_out;
         if (mm >= N) mm -= N;
-        if (o2 < a.c2 && i < a.ci && mm < ns) {
-            float2 v = src[lidx<INNER>(l, n, L, M)];
+        if (tile_line(P, l, o2base, ibase, o1, o2, i) && (mm < ns || mm >= N - a.ns2)) {
+            float2 v = buf[lidx<INNER>(l, n, L, M)];
             if (P.blue) v = cmul(v, P.chirp[n]);
             v.x *= a.scale;
             v.y *= a.scale;
@@ -208,8 +221,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     }
 }
 
-// radix list for a length, largest-first power-of-four grouping; false if a
-// prime factor above 13 remains
+// radix list for a length (fours first, then 2, 3, 5); false if another prime
+// factor remains (such lengths go through Bluestein)
 bool factor_len(int n, int *rad, int &nrad) {
     nrad = 0;
     int r = n;
@@ -217,7 +230,7 @@ bool factor_len(int n, int *rad, int &nrad) {
     while (r % 2 == 0) { r /= 2; ++twos; }
     for (int k = 0; k < twos / 2; ++k) rad[nrad++] = 4;
     if (twos % 2) rad[nrad++] = 2;
-    for (int p : {3, 5, 7, 11, 13}) {
+    for (int p : {3, 5}) {
         while (r % p == 0) { rad[nrad++] = p; r /= p; }
     }
     return r == 1;
@@ -320,18 +333,19 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.bhat = plan.bhat.as<float2>();
     const bool inner = a.ci > 1;
     const int cap = std::max(1, kLdsComplex / plan.m);
-    int L;
     dim3 grid;
     if (inner) {
-        L = std::min({a.ci, cap, 16});
-        grid = dim3((unsigned)((a.ci + L - 1) / L), (unsigned)a.c2, (unsigned)a.c1);
+        P.Li = std::min({a.ci, cap, 16});
+        P.Lo = std::max(1, std::min(a.c2, cap / P.Li));
+        grid = dim3((unsigned)((a.ci + P.Li - 1) / P.Li), (unsigned)((a.c2 + P.Lo - 1) / P.Lo), (unsigned)a.c1);
     } else {
-        L = std::min({a.c2, cap, 32});
-        grid = dim3((unsigned)((a.c2 + L - 1) / L), (unsigned)a.ci, (unsigned)a.c1);
+        P.Li = 1;
+        P.Lo = std::min({a.c2, cap, 32});
+        grid = dim3((unsigned)((a.c2 + P.Lo - 1) / P.Lo), (unsigned)a.ci, (unsigned)a.c1);
     }
+    P.L = P.Li * P.Lo;
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
-    P.L = L;
-    size_t lds = 2 * (size_t)L * plan.m * sizeof(float2);
+    const size_t lds = (size_t)P.L * plan.m * sizeof(float2);
     TimedScope ts(cx, "fft_axis");
     if (inner)
         hipLaunchKernelGGL(k_fft_axis<1>, grid, dim3(kThreads), lds, cx.stream, P);

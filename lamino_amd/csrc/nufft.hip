@@ -95,41 +95,35 @@ struct BinParams {
 __device__ __forceinline__ int wrap0(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ------------------------------------------------------------------ deapodise + pad
-// fine[g][c] = vol_c[i] * corr, where array index i <-> mode i - n/2 <-> fine index mode mod nf.
+// fine[g][c] = vol_c[i] * corr, array index i <-> mode i - n/2 <-> fine index
+// mode mod nf.  Only the mode region is written: the pruned fine-grid FFT
+// never reads the rest.
 template <int NC, bool REAL>
 __global__ __launch_bounds__(256) void k_deapod_pad(float2 *__restrict__ fine, const float *__restrict__ r0,
                                                     const float *__restrict__ r1, const float *__restrict__ r2,
                                                     const float2 *__restrict__ cin, int n1, int n2, int n3,
                                                     int nfz, int nfy, int nfx, const float *__restrict__ cz,
                                                     const float *__restrict__ cy, const float *__restrict__ cxr) {
-    const long long total = (long long)nfz * nfy * nfx;
+    const long long total = (long long)n1 * n2 * n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int gx = (int)(id % nfx);
-        const int gy = (int)((id / nfx) % nfy);
-        const int gz = (int)(id / ((long long)nfx * nfy));
-        const int hz = n1 / 2, hy = n2 / 2, hx = n3 / 2;
-        int iz = gz < n1 - hz ? gz + hz : (gz >= nfz - hz ? gz - nfz + hz : -1);
-        int iy = gy < n2 - hy ? gy + hy : (gy >= nfy - hy ? gy - nfy + hy : -1);
-        int ix = gx < n3 - hx ? gx + hx : (gx >= nfx - hx ? gx - nfx + hx : -1);
-        float2 v[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = make_float2(0.f, 0.f);
-        if (iz >= 0 && iy >= 0 && ix >= 0) {
-            const float cf = cz[iz] * cy[iy] * cxr[ix];
-            const long long vi = ((long long)iz * n2 + iy) * n3 + ix;
-            if (REAL) {
-                v[0].x = r0[vi] * cf;
-                if (NC == 3) {
-                    v[1].x = r1[vi] * cf;
-                    v[2].x = r2[vi] * cf;
-                }
-            } else {
-                v[0] = cscale(cin[vi], cf);
+        const int ix = (int)(id % n3);
+        const int iy = (int)((id / n3) % n2);
+        const int iz = (int)(id / ((long long)n3 * n2));
+        int gz = iz - n1 / 2; if (gz < 0) gz += nfz;
+        int gy = iy - n2 / 2; if (gy < 0) gy += nfy;
+        int gx = ix - n3 / 2; if (gx < 0) gx += nfx;
+        const long long fi = ((long long)gz * nfy + gy) * nfx + gx;
+        const float cf = cz[iz] * cy[iy] * cxr[ix];
+        if constexpr (REAL) {
+            fine[fi * NC] = make_float2(r0[id] * cf, 0.f);
+            if constexpr (NC == 3) {
+                fine[fi * NC + 1] = make_float2(r1[id] * cf, 0.f);
+                fine[fi * NC + 2] = make_float2(r2[id] * cf, 0.f);
             }
+        } else {
+            fine[fi * NC] = cscale(cin[id], cf);
         }
-#pragma unroll
-        for (int c = 0; c < NC; ++c) fine[id * NC + c] = v[c];
     }
 }
 
@@ -668,7 +662,7 @@ void ensure_fine(NufftPlan &pl, int nc) {
 
 void deapod_pad(Ctx &cx, NufftPlan &pl, const float *const m[3], const float2 *cin, int nc) {
     TimedScope ts(cx, "deapod_pad");
-    const long long total = (long long)pl.nf[0] * pl.nf[1] * pl.nf[2];
+    const long long total = pl.n[0] * pl.n[1] * pl.n[2];
     const unsigned grid = grid_for((size_t)total, 256, 1u << 16);
     if (nc == 3)
         hipLaunchKernelGGL((k_deapod_pad<3, true>), dim3(grid), dim3(256), 0, cx.stream, pl.fine.as<float2>(), m[0],
@@ -697,6 +691,50 @@ void crop_deapod(Ctx &cx, NufftPlan &pl, float *const out[3], float2 *cout, int 
                            pl.nf[1], pl.nf[2], pl.corr[0].as<float>(), pl.corr[1].as<float>(),
                            pl.corr[2].as<float>(), scale, 0);
     check_launch();
+}
+
+// Fine-grid 3-D FFT restricted to what the NUFFT needs.  Along an axis with n
+// modes the mode region of the fine grid is the wrapped range
+// [0, n - n/2) U [nf - n/2, nf).
+//  forward (type 2): the input is zero outside the mode region, so the x pass
+//  runs only on (z, y) mode lines and every pass reads only mode entries;
+//  inverse (type 1): only the mode region of the output is kept, so later
+//  passes run on fewer lines and every pass stores only mode entries.
+void fine_fft(Ctx &cx, NufftPlan &pl, int nc, int sign) {
+    const int nfz = pl.nf[0], nfy = pl.nf[1], nfx = pl.nf[2];
+    const int nz = (int)pl.n[0], ny = (int)pl.n[1], nx = (int)pl.n[2];
+    const int hz = nz - nz / 2, hy = ny - ny / 2, hx = nx - nx / 2;   // head counts
+    float2 *f = pl.fine.as<float2>();
+    const long long sX = nc, sY = (long long)nfx * nc, sZ = (long long)nfy * nfx * nc;
+    auto x_pass = [&](bool fwd) {
+        AxisPass p;
+        p.in = p.out = f;
+        p.is1 = p.os1 = sZ; p.is2 = p.os2 = sY; p.isi = p.osi = 1; p.isn = p.osn = sX;
+        p.c1 = nz; p.o1_split = hz; p.o1_gap = nfz - nz;
+        p.c2 = ny; p.o2_split = hy; p.o2_gap = nfy - ny;
+        p.ci = nc;
+        if (fwd) { p.nv = hx; p.nv2 = nx / 2; } else { p.ns = hx; p.ns2 = nx / 2; }
+        run_axis(cx, get_fft_plan(cx, nfx, sign), p);
+    };
+    auto y_pass = [&](bool fwd) {
+        AxisPass p;
+        p.in = p.out = f;
+        p.is1 = p.os1 = sZ; p.isi = p.osi = 1; p.isn = p.osn = sY;
+        p.c1 = nz; p.o1_split = hz; p.o1_gap = nfz - nz;
+        p.c2 = 1; p.ci = nfx * nc;
+        if (fwd) { p.nv = hy; p.nv2 = ny / 2; } else { p.ns = hy; p.ns2 = ny / 2; }
+        run_axis(cx, get_fft_plan(cx, nfy, sign), p);
+    };
+    auto z_pass = [&](bool fwd) {
+        AxisPass p;
+        p.in = p.out = f;
+        p.is2 = p.os2 = sY; p.isi = p.osi = 1; p.isn = p.osn = sZ;
+        p.c1 = 1; p.c2 = nfy; p.ci = nfx * nc;
+        if (fwd) { p.nv = hz; p.nv2 = nz / 2; } else { p.ns = hz; p.ns2 = nz / 2; }
+        run_axis(cx, get_fft_plan(cx, nfz, sign), p);
+    };
+    if (sign < 0) { x_pass(true); y_pass(true); z_pass(true); }
+    else { z_pass(false); y_pass(false); x_pass(false); }
 }
 
 void check_dims(const long long dims[3]) {
@@ -783,7 +821,7 @@ void nufft2_complex(Ctx &cx, Grid &g, const long long dims[3], double eps, float
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
     ensure_fine(pl, 1);
     deapod_pad(cx, pl, nullptr, f, 1);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 1, -1);
+    fine_fft(cx, pl, 1, -1);
     launch_interp(cx, g, pl, c, false);
 }
 
@@ -791,7 +829,7 @@ void nufft1_complex(Ctx &cx, Grid &g, const long long dims[3], double eps, const
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
     ensure_fine(pl, 1);
     launch_spread(cx, g, pl, c, false);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 1, +1);
+    fine_fft(cx, pl, 1, +1);
     crop_deapod(cx, pl, nullptr, f, 1, 1.0f, false);
 }
 
@@ -802,7 +840,7 @@ void forward(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     cx.work[0].ensure(sizeof(float2) * M);
     float2 *cbuf = cx.work[0].as<float2>();
     deapod_pad(cx, pl, m, nullptr, 3);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 3, -1);
+    fine_fft(cx, pl, 3, -1);
     launch_interp(cx, g, pl, cbuf, true);
     det_fft2(cx, cbuf, false, proj, true, (int)g.P, (int)g.R, (int)g.C, +1, 1.0f / (float)(g.R * g.C), cbuf);
 }
@@ -816,7 +854,7 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     float2 *cbuf = cx.work[0].as<float2>();
     det_fft2(cx, proj, true, cbuf, false, (int)g.P, (int)g.R, (int)g.C, -1, 1.0f, cbuf);
     launch_spread(cx, g, pl, cbuf, true);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 3, +1);
+    fine_fft(cx, pl, 3, +1);
     crop_deapod(cx, pl, m, nullptr, 3, 1.0f / (float)(g.R * g.C), accumulate);
 }
 
@@ -828,10 +866,10 @@ void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const f
     cx.work[0].ensure(sizeof(float2) * M);
     float2 *cbuf = cx.work[0].as<float2>();
     deapod_pad(cx, pl, x, nullptr, 3);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 3, -1);
+    fine_fft(cx, pl, 3, -1);
     launch_interp(cx, g, pl, cbuf, true);
     launch_spread(cx, g, pl, cbuf, true);
-    fft3d(cx, pl.fine.as<float2>(), pl.nf[0], pl.nf[1], pl.nf[2], 3, +1);
+    fine_fft(cx, pl, 3, +1);
     crop_deapod(cx, pl, y, nullptr, 3, 1.0f / (float)(g.R * g.C), false);
 }
 
