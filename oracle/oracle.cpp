@@ -740,7 +740,7 @@ template <class T> T dpot(T delta, T sigma_q, T sigma_qp, T p) {
     T ad = std::abs(delta);
     T t1 = (T)(std::pow((double)ad, Q - (double)p) / (double)sigma_qp);
     T t3 = (T)C + t1;
-    if (ad > 0) return (T)(((double)delta / (double)(t3 * sigma_q)) * (Q - ((Q - (double)p) * (double)t1) / (double)t3));
+    if (ad > 0) return (T)((double)(delta / (t3 * sigma_q)) * (Q - ((Q - (double)p) * (double)t1) / (double)t3));
     return (T)(Q / ((double)sigma_q * C));
 }
 
