@@ -37,8 +37,23 @@ def parse():
     ap.add_argument("--cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-recon", action="store_true", help="skip the full reconstruction latency")
-    ap.add_argument("--cpu-projections", type=int, default=180)
+    ap.add_argument("--cpu-projections", type=int, default=24)
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed fwd+adj steps plus a 1 GiB device-to-device copy (PMC calibration), "
+                         "for rocprofv3 runs")
     return ap.parse_args()
+
+
+def hbm_traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, when present."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        return data.get(kernel, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def log(*a):
@@ -98,6 +113,16 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = G * P / ((t1 - t0) / args.steps)
     log(f"fwd+adj: {ms_per_step:.3f} ms/step -> {value:.1f} projections/s")
+    if args.profile:
+        # plain copy kernel of known size: the FETCH_SIZE / WRITE_SIZE calibration
+        nbytes = 1 << 30
+        src, dst = ctx.alloc((nbytes // 4,), np.float32), ctx.alloc((nbytes // 4,), np.float32)
+        src.zero()
+        for _ in range(2):
+            _lib.check(lib.lam_copy_device(ctx.h, ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes))
+        ctx.sync()
+        log(f"copied {nbytes} bytes device-to-device twice")
+        return
 
     kernels = {}
     for name in ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points"):
@@ -131,7 +156,7 @@ def main():
     avg_s = kernels[dom]["avg_ms"] / 1e3
     achieved = bytes_alg / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic_from_profiles(dom), "kernel": dom,
                 "bytes_per_launch": bytes_alg, "avg_launch_ms": round(kernels[dom]["avg_ms"], 4)}
 
     # CG iterations/s: Toeplitz normal operator + ramp preconditioner at the same shape

@@ -59,6 +59,7 @@ lam_status lam_free(lam_ctx *ctx, void *ptr);
 lam_status lam_upload(lam_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 lam_status lam_download(lam_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 lam_status lam_memset(lam_ctx *ctx, void *dst_dev, int value, size_t bytes);
+lam_status lam_copy_device(lam_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
 /* -------------------------------------------------------------------- grids */
 /* PolarGrid<T>(angles, nrows, ncols, gamma), include/polar_grid.h:44-75.

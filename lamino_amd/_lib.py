@@ -51,6 +51,7 @@ _SIGS = {
     "lam_upload": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
     "lam_download": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
     "lam_memset": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, ctypes.c_size_t]),
+    "lam_copy_device": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
     "lam_grid_polar": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, ctypes.c_double, ctypes.POINTER(c_vp)]),
     "lam_grid_polar_pp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, ctypes.POINTER(c_vp)]),
     "lam_grid_destroy": (ctypes.c_int, [c_vp]),

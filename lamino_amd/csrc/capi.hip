@@ -313,6 +313,15 @@ lam_status lam_memset(lam_ctx *ctx, void *dst, int v, size_t bytes) {
     });
 }
 
+lam_status lam_copy_device(lam_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        need(ctx, "ctx");
+        if (!bytes) return;
+        need(dst, "dst"); need(src, "src");
+        LAM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->impl.stream));
+    });
+}
+
 lam_status lam_grid_polar(lam_ctx *ctx, const double *theta, int64_t nproj, int64_t nrows, int64_t ncols, double gamma,
                           lam_grid **out) {
     return guard([&] {
