@@ -64,20 +64,28 @@ __device__ __forceinline__ void unpack(float2 a, float2 b, float2 &X0, float2 &X
     X1 = make_float2(0.5f * t.y, -0.5f * t.x);            // t / (2i)
 }
 
-__device__ __forceinline__ long long mirror(long long id, int L1, int L2, int L3) {
-    const int fx = (int)(id % L3), fy = (int)((id / L3) % L2), fz = (int)(id / ((long long)L3 * L2));
-    const int mx = fx ? L3 - fx : 0, my = fy ? L2 - fy : 0, mz = fz ? L1 - fz : 0;
-    return ((long long)mz * L2 + my) * L3 + mx;
+// The spectra are walked by rows (z, y) of length L3: the row and its mirror row
+// -f are found with one 32-bit division per row, never per element.
+struct SpecRow {
+    long long base, mbase;
+};
+
+__device__ __forceinline__ SpecRow spec_row(int r, int L1, int L2, int L3) {
+    const int z = r / L2, y = r - z * L2;
+    const int mz = z ? L1 - z : 0, my = y ? L2 - y : 0;
+    return {(long long)r * L3, ((long long)mz * L2 + my) * L3};
 }
 
 __global__ __launch_bounds__(256) void k_toep_mul(const float2 *__restrict__ za, const float2 *__restrict__ zb,
                                                   const float2 *__restrict__ K, float2 *__restrict__ wa,
                                                   float2 *__restrict__ wb, int L1, int L2, int L3) {
-    const long long total = (long long)L1 * L2 * L3;
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
-         id += (long long)gridDim.x * blockDim.x) {
+    const int rows = L1 * L2;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+      const SpecRow sr = spec_row(r, L1, L2, L3);
+      for (int x = threadIdx.x; x < L3; x += blockDim.x) {
+        const long long id = sr.base + x;
         float2 X0, X1;
-        unpack(za[id], za[mirror(id, L1, L2, L3)], X0, X1);
+        unpack(za[id], za[sr.mbase + (x ? L3 - x : 0)], X0, X1);
         const float2 X2 = zb[id];
         const float2 k0 = K[3 * id], k1 = K[3 * id + 1], k2 = K[3 * id + 2];
         float2 Y0, Y1, Y2;
@@ -89,6 +97,7 @@ __global__ __launch_bounds__(256) void k_toep_mul(const float2 *__restrict__ za,
         Y2.y = k1.x * X0.y + k2.x * X1.y + k2.y * X2.y;
         wa[id] = make_float2(Y0.x - Y1.y, Y0.y + Y1.x);
         wb[id] = Y2;
+      }
     }
 }
 
@@ -103,11 +112,13 @@ __global__ __launch_bounds__(256) void k_toep_quad(const float2 *__restrict__ za
     double acc[kMaxQuadGrids * 3];
 #pragma unroll
     for (int q = 0; q < kMaxQuadGrids * 3; ++q) acc[q] = 0.0;
-    const long long total = (long long)L1 * L2 * L3;
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
-         id += (long long)gridDim.x * blockDim.x) {
+    const int rows = L1 * L2;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+      const SpecRow sr = spec_row(r, L1, L2, L3);
+      for (int x = threadIdx.x; x < L3; x += blockDim.x) {
+        const long long id = sr.base + x;
         float2 X0, X1;
-        unpack(za[id], za[mirror(id, L1, L2, L3)], X0, X1);
+        unpack(za[id], za[sr.mbase + (x ? L3 - x : 0)], X0, X1);
         const float2 X2 = zb[id];
 #pragma unroll
         for (int g = 0; g < kMaxQuadGrids; ++g) {
@@ -124,6 +135,7 @@ __global__ __launch_bounds__(256) void k_toep_quad(const float2 *__restrict__ za
                 acc[3 * g + 2] += x2x * y2x + x2y * y2y;
             }
         }
+      }
     }
     const int nt = 3 * s.G;
 #pragma unroll
@@ -252,10 +264,13 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
             LAM_HIP(hipMemsetAsync(dst, 0, kbytes, cx.stream));
         }
         const double scale = 1.0 / ((double)gr.R * (double)gr.C * (double)Lt);
-        TimedScope ts(cx, "toep_build");
-        hipLaunchKernelGGL(k_toep_build, dim3(lgrid(e)), dim3(256), 0, cx.stream, dst, gr.geo.as<Geo>(), (int)gr.P,
-                           (int)gr.R, (int)gr.C, (int)dims[0], (int)dims[1], (int)dims[2], L1, L2, L3, scale);
-        check_launch();
+        {
+            TimedScope ts(cx, "toep_build");
+            hipLaunchKernelGGL(k_toep_build, dim3(lgrid(e)), dim3(256), 0, cx.stream, dst, gr.geo.as<Geo>(),
+                               (int)gr.P, (int)gr.R, (int)gr.C, (int)dims[0], (int)dims[1], (int)dims[2], L1, L2, L3,
+                               scale);
+            check_launch();
+        }
         if (per_grid) {
             fft3d(cx, dst, L1, L2, L3, 3, -1);
             hipLaunchKernelGGL(k_add_c, dim3(lgrid(3 * Lt)), dim3(256), 0, cx.stream, t->khat.as<float2>(), dst,
@@ -271,11 +286,13 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
 void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu) {
     Ctx &cx = *t.cx;
     forward_spectra(t, x);
-    const long long Lt = (long long)t.L[0] * t.L[1] * t.L[2];
-    TimedScope ts(cx, "toep_mul");
-    hipLaunchKernelGGL(k_toep_mul, dim3(lgrid(Lt)), dim3(256), 0, cx.stream, t.za.as<float2>(), t.zb.as<float2>(),
-                       t.khat.as<float2>(), t.wa.as<float2>(), t.wb.as<float2>(), t.L[0], t.L[1], t.L[2]);
-    check_launch();
+    {
+        TimedScope ts(cx, "toep_mul");
+        const unsigned rows = (unsigned)std::min<long long>((long long)t.L[0] * t.L[1], 1 << 20);
+        hipLaunchKernelGGL(k_toep_mul, dim3(rows), dim3(256), 0, cx.stream, t.za.as<float2>(), t.zb.as<float2>(),
+                           t.khat.as<float2>(), t.wa.as<float2>(), t.wb.as<float2>(), t.L[0], t.L[1], t.L[2]);
+        check_launch();
+    }
     inverse_spectra(t, y);
     if (mu != 0.f) add_mu_laplacian3(cx, t.n, x, y, mu);
     ++t.applies;

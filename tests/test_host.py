@@ -77,7 +77,6 @@ def test_library_carries_gfx950_code_objects():
 
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
-    assert b"sm_" not in data.split(b"amdgcn")[0][:0] or True
 
 
 def test_context_creation_fails_loudly_without_gpu():
@@ -90,3 +89,33 @@ def test_context_creation_fails_loudly_without_gpu():
 
     with pytest.raises(_lib.LaminoError):
         Context(0)
+
+
+TOMOCAM_HEADERS = ["array.h", "array_ops.h", "bregman.h", "config.h", "device.h", "dtypes.h", "logger.h", "nufft.h",
+                   "optimize.h", "polar_grid.h", "precond.h", "projection.h", "tomocam.h"]
+
+
+@pytest.mark.parametrize("header", TOMOCAM_HEADERS)
+def test_dropin_header_compiles_on_its_own(header):
+    """Each include/tomocam/ header is self-contained C++20 (g++, no HIP headers needed)."""
+    src = f'#include "tomocam/{header}"\nint main() {{ return 0; }}\n'
+    r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_dropin_test_binary_links_and_fails_loudly_without_gpu():
+    """tests/cxx/test_dropin resolves the library and the oracle; without a GPU the
+    first device call raises instead of computing anything on the host."""
+    import torch
+
+    exe = os.path.join(ROOT, "tests", "cxx", "test_dropin")
+    subprocess.run(["make", "-s", "-C", os.path.dirname(exe)], check=True)
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "liblamino_amd.so" in ldd and "liboracle.so" in ldd and "not found" not in ldd, ldd
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    r = subprocess.run([exe, "grid_coords"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "exception" in r.stdout and "FAIL" in r.stdout, r.stdout
