@@ -4,8 +4,8 @@
  * same constructors, indexing, arithmetic and factories.  Host arrays are the
  * interface only; every operator of the reconstruction path runs on the GPU.
  */
-#ifndef TOMOCAM_ARRAY_H
-#define TOMOCAM_ARRAY_H
+#ifndef ARRAY3__H
+#define ARRAY3__H
 
 #include <algorithm>
 #include <complex>
@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "dtypes.h"
+#include "slice.h"
 
 namespace tomocam {
 
@@ -139,4 +140,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_ARRAY_H
+#endif  // ARRAY3__H

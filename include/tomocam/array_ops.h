@@ -3,8 +3,8 @@
  * (include/array_ops.h:39-93 of the reference).  Sums accumulate in double.
  * The solvers' own reductions run on the device (lam_dot, lam_max).
  */
-#ifndef TOMOCAM_ARRAY_OPS_H
-#define TOMOCAM_ARRAY_OPS_H
+#ifndef ARRAY_OPS_H
+#define ARRAY_OPS_H
 
 #include <algorithm>
 #include <cmath>
@@ -60,4 +60,4 @@ namespace tomocam::array {
 
 }  // namespace tomocam::array
 
-#endif  // TOMOCAM_ARRAY_OPS_H
+#endif  // ARRAY_OPS_H

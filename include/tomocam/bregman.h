@@ -4,8 +4,8 @@
  * reference), on the GPU: central differences on interior voxels, zero on
  * every boundary face.
  */
-#ifndef TOMOCAM_BREGMAN_H
-#define TOMOCAM_BREGMAN_H
+#ifndef BREGMAN_H
+#define BREGMAN_H
 
 #include <array>
 
@@ -53,4 +53,4 @@ namespace tomocam::opt {
 
 }  // namespace tomocam::opt
 
-#endif  // TOMOCAM_BREGMAN_H
+#endif  // BREGMAN_H

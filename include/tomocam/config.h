@@ -2,8 +2,8 @@
  * tomocam/config.h -- reconstruction parameters and dataset tuple of the
  * reference (include/config.h:57-58, 143-160), without the TOML reader.
  */
-#ifndef TOMOCAM_CONFIG_H
-#define TOMOCAM_CONFIG_H
+#ifndef CONFIG_H
+#define CONFIG_H
 
 #include <array>
 #include <cstddef>
@@ -37,4 +37,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_CONFIG_H
+#endif  // CONFIG_H

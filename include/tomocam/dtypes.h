@@ -1,10 +1,13 @@
 /*
  * tomocam/dtypes.h -- dims_t, the (n1, n2, n3) shape of a row-major volume
- * (same interface as include/dtypes.h:31-84 of the reference), and Slice<T>,
- * a non-owning view of consecutive n1-slices (include/slice.h:30-63).
+ * (same interface as include/dtypes.h:31-84 of the reference).
+ *
+ * The drop-in headers use the reference headers' include-guard names, so a
+ * translation unit that also reaches a reference header of the same name
+ * (through tiff.h, config.h, ...) gets one definition, whichever comes first.
  */
-#ifndef TOMOCAM_DTYPES_H
-#define TOMOCAM_DTYPES_H
+#ifndef DTYPES__H
+#define DTYPES__H
 
 #include <array>
 #include <cstddef>
@@ -49,23 +52,6 @@ namespace tomocam {
         return os << "dims_t(" << d.n1 << ", " << d.n2 << ", " << d.n3 << ")";
     }
 
-    template <typename T>
-    class Slice {
-        T *data_;
-        dims_t dims_;
-
-      public:
-        Slice(T *data, dims_t d) : data_(data), dims_(d) {}
-        dims_t dims() const { return dims_; }
-        size_t size() const { return dims_.size(); }
-        T *begin() { return data_; }
-        T *end() { return data_ + size(); }
-        const T *begin() const { return data_; }
-        const T *end() const { return data_ + size(); }
-        T &operator[](dims_t i) { return data_[dims_.flat_idx(i.n1, i.n2, i.n3)]; }
-        const T &operator[](dims_t i) const { return data_[dims_.flat_idx(i.n1, i.n2, i.n3)]; }
-    };
-
 }  // namespace tomocam
 
-#endif  // TOMOCAM_DTYPES_H
+#endif  // DTYPES__H

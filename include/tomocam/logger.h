@@ -3,8 +3,8 @@
  * (include/logger.h:28-72 of the reference).  The device solvers do not
  * print; a Logger passed to them is accepted and left unused.
  */
-#ifndef TOMOCAM_LOGGER_H
-#define TOMOCAM_LOGGER_H
+#ifndef LOGGER_H
+#define LOGGER_H
 
 #include <fstream>
 #include <iostream>
@@ -38,4 +38,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_LOGGER_H
+#endif  // LOGGER_H

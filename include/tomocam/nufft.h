@@ -4,8 +4,8 @@
  * PolarGrid's points and a complex volume whose dims (n1, n2, n3) are the
  * modes (k3, k2, k1), FINUFFT mode order 0.  Computed on the GPU in complex64.
  */
-#ifndef TOMOCAM_NUFFT_H
-#define TOMOCAM_NUFFT_H
+#ifndef NUFFT__H
+#define NUFFT__H
 
 #include <complex>
 
@@ -40,4 +40,4 @@ namespace tomocam::nufft {
 
 }  // namespace tomocam::nufft
 
-#endif  // TOMOCAM_NUFFT_H
+#endif  // NUFFT__H

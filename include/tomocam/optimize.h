@@ -8,8 +8,8 @@
  * time (correct, but slow): the MBIR drivers of tomocam.h keep the whole
  * loop on the device instead.
  */
-#ifndef TOMOCAM_OPTIMIZE_H
-#define TOMOCAM_OPTIMIZE_H
+#ifndef TOMOCAM_OPT_H
+#define TOMOCAM_OPT_H
 
 #include <array>
 #include <cmath>
@@ -202,4 +202,4 @@ namespace tomocam::opt {
 
 }  // namespace tomocam::opt
 
-#endif  // TOMOCAM_OPTIMIZE_H
+#endif  // TOMOCAM_OPT_H

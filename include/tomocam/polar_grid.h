@@ -5,8 +5,8 @@
  * fields keep the reference's shape (P, R, C) and values but are materialised
  * on the host only when first read.
  */
-#ifndef TOMOCAM_POLAR_GRID_H
-#define TOMOCAM_POLAR_GRID_H
+#ifndef POLAR_GRID__H
+#define POLAR_GRID__H
 
 #include <memory>
 #include <mutex>
@@ -121,4 +121,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_POLAR_GRID_H
+#endif  // POLAR_GRID__H

@@ -4,8 +4,8 @@
  * computed on the GPU: NUFFT type 2/1 of the three components, the centred
  * 2-D detector transform and the 1/(R*C) scale, as lam_forward/lam_adjoint.
  */
-#ifndef TOMOCAM_PROJECTION_H
-#define TOMOCAM_PROJECTION_H
+#ifndef PROJECTION__H
+#define PROJECTION__H
 
 #include <array>
 #include <cmath>
@@ -79,4 +79,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_PROJECTION_H
+#endif  // PROJECTION__H

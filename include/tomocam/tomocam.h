@@ -8,8 +8,8 @@
  * loop and the crop.  Only the projections go in and the three cropped
  * components come out.
  */
-#ifndef TOMOCAM_TOMOCAM_H
-#define TOMOCAM_TOMOCAM_H
+#ifndef TOMOCAM__H
+#define TOMOCAM__H
 
 #include <array>
 #include <cmath>
@@ -138,4 +138,4 @@ namespace tomocam {
 
 }  // namespace tomocam
 
-#endif  // TOMOCAM_TOMOCAM_H
+#endif  // TOMOCAM__H

@@ -91,7 +91,7 @@ def test_context_creation_fails_loudly_without_gpu():
         Context(0)
 
 
-TOMOCAM_HEADERS = ["array.h", "array_ops.h", "bregman.h", "config.h", "device.h", "dtypes.h", "logger.h", "nufft.h",
+TOMOCAM_HEADERS = ["array.h", "array_ops.h", "bregman.h", "config.h", "device.h", "dtypes.h", "logger.h", "nufft.h", "slice.h",
                    "optimize.h", "polar_grid.h", "precond.h", "projection.h", "tomocam.h"]
 
 
