@@ -250,10 +250,30 @@ __device__ __forceinline__ long long box_to_fine(const TileBox &b, int cell, int
 // plain read-modify-writes, so no cell is ever touched by two lanes or two
 // waves at once and no atomics are needed.  The box is then flushed once to
 // the fine grid with global atomics (neighbouring boxes overlap).
+//
+// The walk is latency-bound unless every LDS access of a point is in flight
+// at once: a point's staged record is laid out so that each lane fetches
+// what it needs (header, values, and both parity candidates of its z and y
+// weights) with a few aligned reads that do not depend on one another, the
+// next point's record is fetched while the current point is accumulated,
+// and all NC accumulator reads of a point are issued before its writes.
 constexpr int kStage = 128;   // points staged per batch
 
+// Staged point record, in 32-bit words: [0] box cell of the point's first
+// kernel cell, [1] parity bits (lz & 1) | (ly & 1) << 1, [4, 4 + 2 NC) the NC
+// complex values, then wz[W], wy[W], wx[W].  Records are 16-byte aligned.
 template <int W, int NC>
-constexpr int stage_words() { return 3 + 3 * W + 2 * NC; }
+struct StageLayout {
+    static constexpr int VQ = 4;
+    static constexpr int WZ = VQ + 2 * NC;
+    static constexpr int WY = WZ + W;
+    static constexpr int WX = WY + W;
+    static constexpr int SW = (WX + W + 3) / 4 * 4;
+    static_assert(W % 2 == 0 && WZ % 2 == 0, "weight pairs must be 8-byte aligned");
+};
+
+template <int W, int NC>
+constexpr int stage_words() { return StageLayout<W, NC>::SW; }
 
 template <int W, bool HALF>
 __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine, const float2 *__restrict__ c,
@@ -262,20 +282,38 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
                                                       BinParams bp, const unsigned *__restrict__ ids,
                                                       const uint4 *__restrict__ chunks) {
     constexpr int NC = HALF ? 3 : 1;
-    constexpr int SW = stage_words<W, NC>();
+    using SL = StageLayout<W, NC>;
+    constexpr int SW = SL::SW;
     constexpr int HW = W / 2;
-    constexpr int NOWN = HW * HW * W;  // cells of one point owned by one wave
+    constexpr int NOWN = HW * HW * W;     // cells of one point owned by one wave
+    constexpr int NCH = (NOWN + 63) / 64; // lane passes over them
     extern __shared__ float lds[];
     const uint4 ch = chunks[blockIdx.x];
     const TileBox bx = tile_box<W>((int)ch.x, bp);
-    float2 *acc = reinterpret_cast<float2 *>(lds);          // [NC][ncell]
-    float *stg = lds + (size_t)2 * NC * bx.ncell;            // [kStage][SW]
+    float *stg = lds;                                                  // [kStage][SW], 16-byte records
+    float2 *acc = reinterpret_cast<float2 *>(lds + (size_t)kStage * SW);  // [NC][ncell]
     for (int i = threadIdx.x; i < NC * bx.ncell; i += blockDim.x) acc[i] = make_float2(0.f, 0.f);
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int zpar = wave >> 1, ypar = wave & 1;
+    const int HYX = bx.HY * bx.HX;
     const long long S = (long long)R * C;
     const long long H = HALF ? (S + 1) / 2 : S;
+
+    // this lane's cells of a point: (a, b, e) = (zoff + 2 ia, yoff + 2 ib, e)
+    int loff[NCH], wzo[NCH], wyo[NCH], wxo[NCH];
+    bool act[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int cidx = lane + 64 * c;
+        act[c] = cidx < NOWN;
+        const int cc = act[c] ? cidx : 0;
+        const int e = cc % W, ib = (cc / W) % HW, ia = cc / (W * HW);
+        loff[c] = (2 * ia * bx.HY + 2 * ib) * bx.HX + e;
+        wzo[c] = SL::WZ + 2 * ia;
+        wyo[c] = SL::WY + 2 * ib;
+        wxo[c] = SL::WX + e;
+    }
 
     for (unsigned base = 0; base < ch.z; base += kStage) {
         const int nb = (int)min((unsigned)kStage, ch.z - base);
@@ -299,9 +337,9 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
             }
             Placed<W> pt;
             place<W>(g, fr, j, k, nfz, nfy, nfx, beta, pt);
-            int lz = wrap0(pt.iz, nfz) - bx.oz, ly = wrap0(pt.iy, nfy) - bx.oy, lx = wrap0(pt.ix, nfx) - bx.ox;
-            float *sp = stg + threadIdx.x * SW;
-            if (!(lz >= 0 && lz < bp.T[0] && ly >= 0 && ly < bp.T[1] && lx >= 0 && lx < bp.T[2])) {
+            const int lz = wrap0(pt.iz, nfz) - bx.oz, ly = wrap0(pt.iy, nfy) - bx.oy, lx = wrap0(pt.ix, nfx) - bx.ox;
+            const bool inside = lz >= 0 && lz < bp.T[0] && ly >= 0 && ly < bp.T[1] && lx >= 0 && lx < bp.T[2];
+            if (!inside) {
                 // a point outside its tile (not expected): spread it straight to the fine grid
 #pragma unroll
                 for (int a = 0; a < W; ++a) {
@@ -323,44 +361,78 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
                         }
                     }
                 }
-                lz = -1;
             }
-            sp[0] = __int_as_float(lz);
-            sp[1] = __int_as_float(ly);
-            sp[2] = __int_as_float(lx);
+            // a point spread directly above stays in the walk as a zero record
+            float *sp = stg + threadIdx.x * SW;
+            int *si = reinterpret_cast<int *>(sp);
+            si[0] = inside ? (lz * bx.HY + ly) * bx.HX + lx : 0;
+            si[1] = inside ? (lz & 1) | ((ly & 1) << 1) : 0;
 #pragma unroll
             for (int m = 0; m < W; ++m) {
-                sp[3 + m] = pt.wz[m];
-                sp[3 + W + m] = pt.wy[m];
-                sp[3 + 2 * W + m] = pt.wx[m];
+                sp[SL::WZ + m] = inside ? pt.wz[m] : 0.f;
+                sp[SL::WY + m] = inside ? pt.wy[m] : 0.f;
+                sp[SL::WX + m] = inside ? pt.wx[m] : 0.f;
             }
 #pragma unroll
             for (int q = 0; q < NC; ++q) {
-                sp[3 + 3 * W + 2 * q] = vc[q].x;
-                sp[3 + 3 * W + 2 * q + 1] = vc[q].y;
+                sp[SL::VQ + 2 * q] = inside ? vc[q].x : 0.f;
+                sp[SL::VQ + 2 * q + 1] = inside ? vc[q].y : 0.f;
             }
         }
         __syncthreads();
-        for (int kk = 0; kk < nb; ++kk) {
+
+        // one point's record as this lane needs it
+        struct Rec {
+            int2 hdr;
+            float2 v[NC];
+            float2 wz[NCH], wy[NCH];   // parity candidates (w[2i], w[2i + 1])
+            float wx[NCH];
+        };
+        auto fetch = [&](int kk, Rec &r) {
             const float *sp = stg + kk * SW;
-            const int lz = __float_as_int(sp[0]);
-            if (lz < 0) continue;
-            const int ly = __float_as_int(sp[1]), lx = __float_as_int(sp[2]);
-            const int zoff = (zpar - lz) & 1, yoff = (ypar - ly) & 1;
-            float2 vq[NC];
+            r.hdr = *reinterpret_cast<const int2 *>(sp);
 #pragma unroll
-            for (int q = 0; q < NC; ++q) vq[q] = make_float2(sp[3 + 3 * W + 2 * q], sp[3 + 3 * W + 2 * q + 1]);
-            for (int cidx = lane; cidx < NOWN; cidx += 64) {
-                const int e = cidx % W, b = yoff + 2 * ((cidx / W) % HW), a = zoff + 2 * (cidx / (W * HW));
-                const float wgt = sp[3 + a] * sp[3 + W + b] * sp[3 + 2 * W + e];
-                const int cell = ((lz + a) * bx.HY + (ly + b)) * bx.HX + (lx + e);
+            for (int q = 0; q < NC; ++q) r.v[q] = *reinterpret_cast<const float2 *>(sp + SL::VQ + 2 * q);
 #pragma unroll
-                for (int q = 0; q < NC; ++q) {
-                    float2 t = acc[q * bx.ncell + cell];
-                    t.x += vq[q].x * wgt;
-                    t.y += vq[q].y * wgt;
-                    acc[q * bx.ncell + cell] = t;
+            for (int c = 0; c < NCH; ++c) {
+                r.wz[c] = *reinterpret_cast<const float2 *>(sp + wzo[c]);
+                r.wy[c] = *reinterpret_cast<const float2 *>(sp + wyo[c]);
+                r.wx[c] = sp[wxo[c]];
+            }
+        };
+        auto accumulate = [&](const Rec &r) {
+            const int cb = __builtin_amdgcn_readfirstlane(r.hdr.x);
+            const int bits = __builtin_amdgcn_readfirstlane(r.hdr.y);
+            const int zoff = (zpar ^ bits) & 1, yoff = (ypar ^ (bits >> 1)) & 1;
+            const int cell0 = cb + (zoff ? HYX : 0) + (yoff ? bx.HX : 0);
+            float2 t[NCH][NC];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                if (act[c]) {
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) t[c][q] = acc[q * bx.ncell + cell0 + loff[c]];
                 }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                if (act[c]) {
+                    const float w = (zoff ? r.wz[c].y : r.wz[c].x) * (yoff ? r.wy[c].y : r.wy[c].x) * r.wx[c];
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) {
+                        t[c][q].x += r.v[q].x * w;
+                        t[c][q].y += r.v[q].y * w;
+                        acc[q * bx.ncell + cell0 + loff[c]] = t[c][q];
+                    }
+                }
+        };
+        // two records in flight: the next one is fetched before the current one is accumulated
+        Rec ra, rb;
+        fetch(0, ra);
+        for (int kk = 0; kk < nb; kk += 2) {
+            if (kk + 1 < nb) fetch(kk + 1, rb);
+            accumulate(ra);
+            if (kk + 1 < nb) {
+                if (kk + 2 < nb) fetch(kk + 2, ra);
+                accumulate(rb);
             }
         }
     }
@@ -530,7 +602,8 @@ constexpr size_t kLdsBudget = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
 
 size_t spread_lds_bytes(int T, int w, int nc) {
     const size_t h = (size_t)(T + w - 1);
-    return h * h * h * nc * 8 + (size_t)kStage * (3 + 3 * w + 2 * nc) * 4;
+    const size_t sw = (size_t)(4 + 2 * nc + 3 * w + 3) / 4 * 4;   // StageLayout<w, nc>::SW
+    return h * h * h * nc * 8 + (size_t)kStage * sw * 4;
 }
 
 BinParams bin_params(const NufftPlan &pl, int nc) {
