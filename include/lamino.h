@@ -159,6 +159,14 @@ lam_status lam_lipschitz(lam_ctx *ctx, const int64_t dims[3], lam_op_fn A, void 
  * whose operators are std::function closures: src/mbir_cg.cpp:96-105.) */
 lam_status lam_toep_op(void *user, const float *const x[3], float *const y[3]);
 
+/* Spreader/interpolator autotuner (no reference counterpart: FINUFFT picks its
+ * bins internally).  Times candidate tile shapes (z, y, x) of this grid's
+ * three-component spread + interpolation on the fine grid of dims and keeps
+ * the fastest for every later call with the same fine grid and detector, in
+ * this process.  tile_out: the chosen shape; best_ms: its spread + interp time. */
+lam_status lam_autotune_bins(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, int tile_out[3],
+                             double *best_ms);
+
 /* ---------------------------------------------------------- instrumentation */
 /* HIP-event timing of the library's own kernels on the context stream.
  * Enabling clears earlier records.  (No reference counterpart; the reference
@@ -199,6 +207,57 @@ typedef struct {
  * out3_host receives the 3 cropped components [3][n1][n2][n3]. */
 lam_status lam_mbir(lam_ctx *ctx, const lam_dataset *datasets, int ndatasets, const int64_t recon_dims[3],
                     const lam_recon_params *params, float *out3_host, lam_recon_stats *stats);
+
+/* Upper bound of the device memory one lam_mbir job of these shapes allocates. */
+lam_status lam_mbir_footprint(const lam_dataset *datasets, int ndatasets, const int64_t recon_dims[3],
+                              const lam_recon_params *params, size_t *bytes);
+
+/* ------------------------------------------------------- batched / served jobs
+ * (No counterpart in the reference, which runs one job per process,
+ * src/recon.cpp:65-73.)  A server owns `slots` worker threads on one device,
+ * each with its own stream, and admits submitted jobs in arrival order as a
+ * slot frees and the HBM budget allows (lam_mbir_footprint per job).  Jobs on
+ * different slots overlap on the device; each job keeps its iterate, Bregman
+ * auxiliaries and Toeplitz kernels resident until it finishes.  A job's
+ * result does not depend on what runs beside it. */
+typedef struct lam_server lam_server;
+
+enum {
+    LAM_SERVE_PLAIN = 0,   /* the lam_mbir solver loop (host reads per stopping test) */
+    LAM_SERVE_ASYNC = 1,   /* solver scalars and stopping tests on the device */
+    LAM_SERVE_GRAPH = 2    /* ASYNC, with each iteration replayed from a captured hipGraph */
+};
+
+typedef struct {
+    int64_t id;
+    int status;                /* LAM_OK or the job's error code */
+    int slot;                  /* worker that ran it */
+    double submit_s;           /* seconds since the server was created */
+    double start_s;            /* admitted */
+    double end_s;              /* result written to out3_host */
+    size_t footprint_est;      /* admission estimate (bytes) */
+    size_t footprint_peak;     /* measured peak of the job's device buffers (bytes) */
+    lam_recon_stats stats;
+} lam_job_info;
+
+typedef struct {
+    int64_t submitted, completed, failed, running, queued;
+    size_t budget, in_use, peak_in_use;
+    int slots, mode;
+} lam_server_stats;
+
+lam_status lam_server_create(int device, size_t hbm_budget, int slots, int mode, lam_server **out);
+/* Finishes every queued job, then stops the workers. */
+lam_status lam_server_destroy(lam_server *server);
+/* Queues a job; projections and angles are copied, out3_host must stay valid
+ * until lam_server_wait returns for the job.  Rejects (LAM_ERR_BUDGET) a job
+ * whose footprint exceeds the whole budget. */
+lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, int ndatasets,
+                             const int64_t recon_dims[3], const lam_recon_params *params, float *out3_host,
+                             int64_t *job_id);
+/* Blocks until the job has finished; its error, if any, is the return value. */
+lam_status lam_server_wait(lam_server *server, int64_t job_id, lam_job_info *info);
+lam_status lam_server_query(lam_server *server, lam_server_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,21 @@ class lam_recon_stats(ctypes.Structure):
                 ("cg_iters", c_i64), ("outer_iters", c_i64)]
 
 
+class lam_job_info(ctypes.Structure):
+    _fields_ = [("id", c_i64), ("status", ctypes.c_int), ("slot", ctypes.c_int), ("submit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("end_s", ctypes.c_double), ("footprint_est", ctypes.c_size_t),
+                ("footprint_peak", ctypes.c_size_t), ("stats", lam_recon_stats)]
+
+
+class lam_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", c_i64), ("completed", c_i64), ("failed", c_i64), ("running", c_i64),
+                ("queued", c_i64), ("budget", ctypes.c_size_t), ("in_use", ctypes.c_size_t),
+                ("peak_in_use", ctypes.c_size_t), ("slots", ctypes.c_int), ("mode", ctypes.c_int)]
+
+
+LAM_SERVE_PLAIN, LAM_SERVE_ASYNC, LAM_SERVE_GRAPH = 0, 1, 2
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "lam_last_error": (ctypes.c_char_p, []),
@@ -85,10 +100,20 @@ _SIGS = {
                                   ctypes.c_double, ctypes.c_double, c_i64]),
     "lam_lipschitz": (ctypes.c_int, [c_vp, c_i64p, OP_FN, c_vp, c_i64, ctypes.c_double, ctypes.c_uint64, c_dp]),
     "lam_toep_op": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "lam_autotune_bins": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int), c_dp]),
     "lam_timing_enable": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "lam_timing_query": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_dp, c_i64p]),
     "lam_mbir": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
                                 ctypes.POINTER(lam_recon_params), c_vp, ctypes.POINTER(lam_recon_stats)]),
+    "lam_mbir_footprint": (ctypes.c_int, [ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
+                                          ctypes.POINTER(lam_recon_params), ctypes.POINTER(ctypes.c_size_t)]),
+    "lam_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(c_vp)]),
+    "lam_server_destroy": (ctypes.c_int, [c_vp]),
+    "lam_server_submit": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
+                                         ctypes.POINTER(lam_recon_params), c_vp, c_i64p]),
+    "lam_server_wait": (ctypes.c_int, [c_vp, c_i64, ctypes.POINTER(lam_job_info)]),
+    "lam_server_query": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_server_stats)]),
 }
 
 _lib = None

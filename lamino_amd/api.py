@@ -249,6 +249,15 @@ def forward(magnetization, pg: PolarGrid, gamma: float | None = None, eps: float
     return out.download()
 
 
+def autotune_bins(pg: PolarGrid, recon_dims, eps: float = DEFAULT_EPS):
+    """Times spreading tile shapes for this grid and volume and keeps the fastest
+    for later calls with the same shapes; returns ((Tz, Ty, Tx), spread+interp ms)."""
+    T = (ctypes.c_int * 3)()
+    ms = ctypes.c_double()
+    check(pg.ctx.lib.lam_autotune_bins(pg.ctx.h, pg.h, _dims(recon_dims), float(eps), T, ctypes.byref(ms)))
+    return tuple(T), ms.value
+
+
 def adjoint(proj, pg: PolarGrid, recon_dims, gamma: float | None = None, eps: float = DEFAULT_EPS) -> np.ndarray:
     """src/projection.cpp:89-126; projections (P, R, C) -> (3, n1, n2, n3)."""
     ctx = pg.ctx
@@ -511,8 +520,8 @@ class ReconParams:
     eps: float = DEFAULT_EPS
 
 
-def _mbir(method: int, datasets, recon_dims, params: ReconParams, ctx: Context | None = None, stats=False):
-    ctx = ctx or default_context()
+def _pack_job(method: int, datasets, params: ReconParams):
+    """lam_dataset array and lam_recon_params of a job, plus the arrays they point into."""
     keep = []
     arr = (_lib.lam_dataset * len(datasets))()
     for i, ds in enumerate(datasets):
@@ -532,6 +541,20 @@ def _mbir(method: int, datasets, recon_dims, params: ReconParams, ctx: Context |
             arr[i].gamma = ga.ctypes.data_as(ctypes.c_void_p)
     pr = _lib.lam_recon_params(method, params.max_iters, params.inner_iters, params.lam, params.mu, params.sigma,
                                params.p, params.tol, params.xtol, params.pad_factor, params.eps)
+    return arr, pr, keep
+
+
+def mbir_footprint(method: int, datasets, recon_dims, params: ReconParams) -> int:
+    """Upper bound of the device bytes one job of these shapes allocates."""
+    arr, pr, _keep = _pack_job(method, datasets, params)
+    out = ctypes.c_size_t()
+    check(_lib.load().lam_mbir_footprint(arr, len(datasets), _dims(recon_dims), ctypes.byref(pr), ctypes.byref(out)))
+    return out.value
+
+
+def _mbir(method: int, datasets, recon_dims, params: ReconParams, ctx: Context | None = None, stats=False):
+    ctx = ctx or default_context()
+    arr, pr, _keep = _pack_job(method, datasets, params)
     out = np.empty((3,) + tuple(recon_dims), np.float32)
     st = _lib.lam_recon_stats()
     check(ctx.lib.lam_mbir(ctx.h, arr, len(datasets), _dims(recon_dims), ctypes.byref(pr),

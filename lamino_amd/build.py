@@ -2,7 +2,6 @@
 
 Compiles every HIP source under lamino_amd/csrc with hipcc for gfx950 and
 links them into one shared library exporting the C ABI of include/lamino.h.
-Also builds the C++ drop-in layer (include/tomocam/) test driver when asked.
 """
 from __future__ import annotations
 
@@ -31,7 +30,7 @@ COMMON_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["fft.hip", "nufft.hip", "ops.hip", "toeplitz.hip", "solvers.hip", "capi.hip"]
+SOURCES = ["fft.hip", "nufft.hip", "ops.hip", "toeplitz.hip", "solvers.hip", "capi.hip", "serve.hip"]
 
 
 def _compile(src: str) -> str:

@@ -7,6 +7,7 @@
 #include <limits>
 #include <new>
 
+#include "capi_util.h"
 #include "fft.h"
 #include "nufft.h"
 #include "ops.h"
@@ -20,36 +21,6 @@ Ctx::~Ctx() {
 }
 
 namespace {
-
-thread_local std::string g_last_error;
-
-template <class F> lam_status guard(F &&f) {
-    try {
-        f();
-        return LAM_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &e) {
-        g_last_error = e.what();
-        return LAM_ERR_NOMEM;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return LAM_ERR_ARG;
-    }
-}
-
-void need(const void *p, const char *what) {
-    if (!p) throw Error(LAM_ERR_ARG, std::string("null argument: ") + what);
-}
-
-void dims_of(const int64_t *d, long long out[3]) {
-    need(d, "dims");
-    for (int a = 0; a < 3; ++a) {
-        if (d[a] < 1) throw Error(LAM_ERR_ARG, "dimensions must be positive");
-        out[a] = d[a];
-    }
-}
 
 unsigned vgrid(long long n) { return grid_for((size_t)n, 256, 1u << 16); }
 
@@ -88,7 +59,7 @@ using clk = std::chrono::steady_clock;
 
 // Runs one reconstruction job end to end on the context's stream.
 void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr, float *out3,
-              lam_recon_stats *stats) {
+              lam_recon_stats *stats, MbirMode mode) {
     LAM_REQUIRE(nds >= 1, LAM_ERR_ARG, "MBIR needs at least one dataset");
     LAM_REQUIRE(pr.method >= 1 && pr.method <= 3, LAM_ERR_ARG, "unknown reconstruction method");
     const auto t0 = clk::now();
@@ -125,6 +96,9 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
         std::vector<double> gam(D.nproj);
         for (long long p = 0; p < D.nproj; ++p) gam[p] = D.gamma ? D.gamma[p] : D.gamma_const;
         grids.push_back(make_grid(cx, D.alpha, gam.data(), D.nproj, R2, C2));
+        // the graph form picks its spreading tiles by measurement, once per shape
+        if (mode == MbirMode::Graph && !bins_tuned(cx, *grids.back(), od, eps))
+            autotune_bins(cx, *grids.back(), od, eps, nullptr);
         float *ys[3] = {yT.v().c[0], yT.v().c[1], yT.v().c[2]};
         adjoint(cx, *grids.back(), od, eps, padded.as<float>(), ys, true);
     }
@@ -141,15 +115,27 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
     LAM_HIP(hipMemsetAsync(x.buf.ptr, 0, x.buf.bytes, cx.stream));
     CgStats st;
     long long outer = 0;
+    long long normal_ops = -1;
     if (pr.method == 1) {
         OpFn A = [&](CVol3 u, Vol3 y) { toep_apply(*toep, u, y, 0.f); };
-        cg_solve(cx, od, A, yT.cv(), x.cv(), x.v(), pr.max_iters, pr.tol, &st);
+        if (mode == MbirMode::Graph || mode == MbirMode::Async) {
+            cg_graph_solve(cx, od, A, yT.cv(), x.v(), pr.max_iters, pr.tol, mode == MbirMode::Graph, &st);
+            normal_ops = st.applies;
+        } else {
+            cg_solve(cx, od, A, yT.cv(), x.cv(), x.v(), pr.max_iters, pr.tol, &st);
+        }
         outer = 1;
     } else if (pr.method == 2) {
         const float mu = (float)pr.mu;
         OpFn A = [&](CVol3 u, Vol3 y) { toep_apply(*toep, u, y, mu); };
-        split_bregman(cx, od, A, true, yT.cv(), x.cv(), pr.lambda, pr.mu, pr.max_iters, pr.inner_iters, pr.tol,
-                      pr.xtol, x.v(), &st, &outer);
+        if (mode == MbirMode::Graph || mode == MbirMode::Async) {
+            split_bregman_graph(cx, od, A, yT.cv(), x.cv(), pr.lambda, pr.mu, pr.max_iters, pr.inner_iters, pr.tol,
+                                pr.xtol, x.v(), mode == MbirMode::Graph, &st, &outer);
+            normal_ops = st.applies;
+        } else {
+            split_bregman(cx, od, A, true, yT.cv(), x.cv(), pr.lambda, pr.mu, pr.max_iters, pr.inner_iters, pr.tol,
+                          pr.xtol, x.v(), &st, &outer);
+        }
     } else {
         const float G = (float)nds;
         const float sigma = (float)pr.sigma, p = (float)pr.p;
@@ -195,10 +181,53 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
     if (stats) {
         stats->setup_s = std::chrono::duration<double>(t1 - t0).count();
         stats->solve_s = std::chrono::duration<double>(t2 - t1).count();
-        stats->normal_ops = toep->applies;
+        stats->normal_ops = normal_ops >= 0 ? normal_ops : toep->applies;
         stats->cg_iters = st.iters;
         stats->outer_iters = outer;
     }
+}
+
+// Upper bound of the device bytes mbir_run allocates for a job: every buffer
+// of the setup (projections, point bins, fine grid, detector spectra) counted
+// as if alive together with the Toeplitz operator and the solver state.
+size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr) {
+    LAM_REQUIRE(nds >= 1, LAM_ERR_ARG, "MBIR needs at least one dataset");
+    LAM_REQUIRE(pr.method >= 1 && pr.method <= 3, LAM_ERR_ARG, "unknown reconstruction method");
+    const float PAD = (float)pr.pad_factor;
+    const float dpad = pr.method == 1 ? (float)((double)PAD - 1.0) : PAD;
+    const float vpad = pr.method == 1 ? dpad : (float)(PAD - 1.0f);
+    long long od[3];
+    for (int a = 0; a < 3; ++a) od[a] = pad_n(rd[a], vpad);
+    const size_t n = (size_t)(od[0] * od[1] * od[2]);
+    const double eps = pr.eps > 0 ? pr.eps : 1e-5;
+    size_t nf = 1, lt = 1, tiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        const size_t f = nufft_fine_len(od[a], eps);
+        nf *= f;
+        tiles *= (f + 1) / 2;                       // bins are never smaller than 2 cells
+        lt *= (size_t)toep_len(od[a]);
+    }
+    size_t raw = 0, padded = 0, spectra = 0, bintmp = 0, bins = 0;
+    for (int g = 0; g < nds; ++g) {
+        const lam_dataset &D = ds[g];
+        const size_t R2 = (size_t)pad_n(D.nrows, dpad), C2 = (size_t)pad_n(D.ncols, dpad);
+        const size_t P = (size_t)D.nproj, M = P * R2 * C2, half = P * ((R2 * C2 + 1) / 2);
+        raw = std::max(raw, 4 * P * (size_t)D.nrows * (size_t)D.ncols);
+        padded = std::max(padded, 4 * M);
+        spectra = std::max(spectra, 8 * M);
+        bintmp = std::max(bintmp, 8 * half + 8 * tiles);      // tile ids, ranks, counts, offsets
+        // sorted ids, chunk list, geometry; twice: the autotuner holds two tile shapes at once
+        bins += 2 * (4 * half + 16 * (half / 1024 + tiles)) + 128 * P;
+    }
+    const size_t fine = 24 * nf;                                   // 3 complex components
+    const size_t toep = 24 * lt * (1 + (pr.method == 3 ? (size_t)nds : 0)) + 32 * lt;
+    size_t solve = 12 * n;                                         // x
+    if (pr.method == 1) solve += 48 * n + 8 * n;                   // r z p Ap + preconditioner
+    if (pr.method == 2) solve += 24 * n + 72 * n + 48 * n + 8 * n; // xold rhs, d b, PCG
+    if (pr.method == 3) solve += 36 * n;                           // xold y g
+    const size_t crop = 4 * (size_t)(rd[0] * rd[1] * rd[2]);
+    const size_t slack = 64ull << 20;                              // plans, reductions, twiddles
+    return 12 * n + raw + padded + spectra + bintmp + bins + fine + toep + solve + crop + slack;
 }
 
 }  // namespace lam
@@ -665,13 +694,25 @@ lam_status lam_timing_query(lam_ctx *ctx, const char *name, double *total_ms, in
     });
 }
 
+lam_status lam_autotune_bins(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, int tile_out[3],
+                             double *best_ms) {
+    return guard([&] {
+        need(ctx, "ctx"); need(grid, "grid");
+        long long d[3];
+        dims_of(dims, d);
+        const auto T = autotune_bins(ctx->impl, *grid->impl, d, eps > 0 ? eps : 1e-5, best_ms);
+        if (tile_out)
+            for (int a = 0; a < 3; ++a) tile_out[a] = T[a];
+    });
+}
+
 lam_status lam_mbir(lam_ctx *ctx, const lam_dataset *datasets, int nds, const int64_t recon_dims[3],
                     const lam_recon_params *params, float *out3, lam_recon_stats *stats) {
     return guard([&] {
         need(ctx, "ctx"); need(datasets, "datasets"); need(params, "params"); need(out3, "out3");
         long long rd[3];
         dims_of(recon_dims, rd);
-        mbir_run(ctx->impl, datasets, nds, rd, *params, out3, stats);
+        mbir_run(ctx->impl, datasets, nds, rd, *params, out3, stats, MbirMode::Plain);
     });
 }
 

@@ -8,9 +8,11 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -39,30 +41,74 @@ struct Error : std::runtime_error {
     } while (0)
 
 // ---------------------------------------------------------------- device buffer
+// Device bytes held by the buffers a host thread allocated: current and peak.
+// The job server runs one job at a time per worker thread and reads the peak
+// as the job's measured footprint.  (Never freed: buffers may outlive the
+// thread-exit order of thread_local objects.)
+struct MemTally {
+    size_t cur = 0, peak = 0;
+};
+inline MemTally *mem_tally() {
+    thread_local MemTally *t = new MemTally;
+    return t;
+}
+
+// Graph capture (GraphStep) and device allocation exclude each other across
+// threads: hipMalloc / hipFree may synchronise the device, which must not meet
+// a stream that another thread is capturing.  Allocating on a thread that is
+// itself capturing is a bug and raises instead of waiting on itself.
+inline std::shared_mutex &capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+inline bool &capturing_here() {
+    thread_local bool c = false;
+    return c;
+}
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    MemTally *tally = nullptr;
     DevBuf() = default;
     explicit DevBuf(size_t b) { alloc(b); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), tally(o.tally) {
+        o.ptr = nullptr; o.bytes = 0; o.tally = nullptr;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            ptr = o.ptr; bytes = o.bytes; tally = o.tally;
+            o.ptr = nullptr; o.bytes = 0; o.tally = nullptr;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t b) {
         release();
-        if (b) LAM_HIP(hipMalloc(&ptr, b));
+        if (b) {
+            if (capturing_here()) throw Error(LAM_ERR_STATE, "device allocation inside a graph capture");
+            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            LAM_HIP(hipMalloc(&ptr, b));
+            tally = mem_tally();
+            tally->cur += b;
+            if (tally->cur > tally->peak) tally->peak = tally->cur;
+        }
         bytes = b;
     }
     // grow-only
     void ensure(size_t b) { if (b > bytes) alloc(b); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            (void)hipFree(ptr);
+            if (tally) tally->cur -= bytes;
+        }
         ptr = nullptr;
         bytes = 0;
+        tally = nullptr;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
@@ -101,6 +147,8 @@ struct Ctx {
     bool own_stream = false;
     std::map<std::pair<int, int>, std::unique_ptr<FftPlan>> fft_plans;   // (N, sign)
     std::map<std::tuple<long long, long long, long long, int>, std::unique_ptr<NufftPlan>> nufft_plans;
+    // spreader/interpolator tile shape per (fine grid, width, components, detector), from the autotuner
+    std::map<std::vector<long long>, std::array<int, 3>> bin_tune;
     // general scratch used by a single call at a time
     DevBuf red;        // FP64 reduction partials
     DevBuf scal;       // small device scalar block
@@ -110,6 +158,14 @@ struct Ctx {
 };
 
 FftPlan &get_fft_plan(Ctx &cx, int n, int sign);
+
+// Host -> device copy ordered on the context stream and complete on return.
+// It never touches the legacy stream, so it is safe while another thread of
+// the process is capturing a graph.
+inline void upload_now(Ctx &cx, void *dst, const void *src, size_t bytes) {
+    LAM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+}
 
 // Records start/stop events around the launches made in its scope when the
 // context's timer is on.

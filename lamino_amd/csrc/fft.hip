@@ -277,7 +277,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         roots[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     plan->roots.alloc(sizeof(float2) * m);
-    LAM_HIP(hipMemcpy(plan->roots.ptr, roots.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+    upload_now(cx, plan->roots.ptr, roots.data(), sizeof(float2) * m);
     if (plan->blue) {
         std::vector<std::complex<double>> ch(n), b(m, 0.0), bh(m);
         std::vector<float2> chf(n), bhf(m);
@@ -310,8 +310,8 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         }
         plan->chirp.alloc(sizeof(float2) * n);
         plan->bhat.alloc(sizeof(float2) * m);
-        LAM_HIP(hipMemcpy(plan->chirp.ptr, chf.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-        LAM_HIP(hipMemcpy(plan->bhat.ptr, bhf.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+        upload_now(cx, plan->chirp.ptr, chf.data(), sizeof(float2) * n);
+        upload_now(cx, plan->bhat.ptr, bhf.data(), sizeof(float2) * m);
     }
     FftPlan &ref = *plan;
     cx.fft_plans.emplace(key, std::move(plan));

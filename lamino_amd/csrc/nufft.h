@@ -2,6 +2,8 @@
 // upsampling 2) specialised to the plane-per-projection point sets of the
 // reference's PolarGrid (include/polar_grid.h:44-75).
 #pragma once
+#include <algorithm>
+
 #include "common.h"
 #include "fft.h"
 
@@ -49,6 +51,8 @@ struct NufftPlan {
 };
 
 int width_for_eps(double eps);
+// fine-grid length of one axis of n modes at tolerance eps (the plan's rule)
+size_t nufft_fine_len(long long n, double eps);
 NufftPlan &get_nufft_plan(Ctx &cx, const long long dims[3], double eps);
 std::unique_ptr<Grid> make_grid(Ctx &cx, const double *alpha, const double *gamma, long long P, long long R,
                                 long long C);
@@ -61,6 +65,11 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
              bool accumulate);
 void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const float *const x[3],
                   float *const y[3]);
+// Picks the spreading/interpolation tile shape of this grid and fine grid by
+// timing candidates; the choice is kept in cx.bin_tune.  best_ms: spread+interp time.
+std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms);
+// whether a tuned shape exists (in this context or the process-wide table)
+bool bins_tuned(Ctx &cx, Grid &g, const long long dims[3], double eps);
 
 }  // namespace lam
 

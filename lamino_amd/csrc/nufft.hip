@@ -10,6 +10,7 @@
 //     c(q) + conj(c(-q)) over the half set (exact, no symmetry assumed).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "nufft.h"
 
@@ -600,23 +601,69 @@ constexpr unsigned kChunk = 2048;  // points per spreading workgroup
 
 constexpr size_t kLdsBudget = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
 
-size_t spread_lds_bytes(int T, int w, int nc) {
-    const size_t h = (size_t)(T + w - 1);
-    const size_t sw = (size_t)(4 + 2 * nc + 3 * w + 3) / 4 * 4;   // StageLayout<w, nc>::SW
-    return h * h * h * nc * 8 + (size_t)kStage * sw * 4;
+size_t box_cells(const int T[3], int w) {
+    return (size_t)(T[0] + w - 1) * (size_t)(T[1] + w - 1) * (size_t)(T[2] + w - 1);
 }
 
-BinParams bin_params(const NufftPlan &pl, int nc) {
+size_t spread_lds_bytes(const int T[3], int w, int nc) {
+    const size_t sw = (size_t)(4 + 2 * nc + 3 * w + 3) / 4 * 4;   // StageLayout<w, nc>::SW
+    return box_cells(T, w) * nc * 8 + (size_t)kStage * sw * 4;
+}
+
+size_t interp_lds_bytes(const int T[3], int w, int nc) { return box_cells(T, w) * nc * 8; }
+
+BinParams make_bins(const int T[3], const NufftPlan &pl) {
     BinParams bp;
-    int T = 2;
-    for (int cand : {16, 12, 10, 8, 6, 4, 2}) {
-        if (spread_lds_bytes(cand, pl.w, nc) <= kLdsBudget) { T = cand; break; }
-    }
     for (int a = 0; a < 3; ++a) {
-        bp.T[a] = T;
-        bp.nt[a] = (pl.nf[a] + T - 1) / T;
+        bp.T[a] = T[a];
+        bp.nt[a] = (pl.nf[a] + T[a] - 1) / T[a];
     }
     return bp;
+}
+
+std::vector<long long> tune_key(const Grid &g, const NufftPlan &pl, int nc) {
+    return {pl.nf[0], pl.nf[1], pl.nf[2], pl.w, nc, g.R, g.C};
+}
+
+bool bins_fit(const int T[3], const NufftPlan &pl, int nc) {
+    for (int a = 0; a < 3; ++a)
+        if (T[a] < 2 || T[a] > pl.nf[a]) return false;
+    return spread_lds_bytes(T, pl.w, nc) <= kLdsBudget && interp_lds_bytes(T, pl.w, nc) <= kLdsBudget;
+}
+
+// Tile shapes chosen by the autotuner, shared by every context of the process
+// (a shape tuned by one server worker serves the others).
+std::mutex g_tune_mu;
+std::map<std::vector<long long>, std::array<int, 3>> g_tune;
+
+bool lookup_tuned(Ctx &cx, const std::vector<long long> &key, std::array<int, 3> &T) {
+    auto it = cx.bin_tune.find(key);
+    if (it != cx.bin_tune.end()) {
+        T = it->second;
+        return true;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto jt = g_tune.find(key);
+    if (jt == g_tune.end()) return false;
+    T = jt->second;
+    cx.bin_tune[key] = T;
+    return true;
+}
+
+// The tuned tile shape of this grid and fine grid when there is one, else the
+// largest cubic tile whose spreading box fits the LDS budget.
+BinParams bin_params(Ctx &cx, const Grid &g, const NufftPlan &pl, int nc) {
+    std::array<int, 3> tuned;
+    if (lookup_tuned(cx, tune_key(g, pl, nc), tuned)) return make_bins(tuned.data(), pl);
+    int T[3] = {2, 2, 2};
+    for (int cand : {16, 12, 10, 8, 6, 4, 2}) {
+        const int c3[3] = {cand, cand, cand};
+        if (spread_lds_bytes(c3, pl.w, nc) <= kLdsBudget) {
+            T[0] = T[1] = T[2] = cand;
+            break;
+        }
+    }
+    return make_bins(T, pl);
 }
 
 template <class K>
@@ -682,7 +729,7 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
 template <int W, bool HALF>
 void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams &bp, SortedPoints &sp) {
     constexpr int NC = HALF ? 3 : 1;
-    const size_t lds = spread_lds_bytes(bp.T[0], W, NC);
+    const size_t lds = spread_lds_bytes(bp.T, W, NC);
     allow_lds(&k_spread_tiled<W, HALF>, lds);
     hipLaunchKernelGGL((k_spread_tiled<W, HALF>), dim3((unsigned)sp.nchunks), dim3(256), lds, cx.stream,
                        pl.fine.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2],
@@ -692,8 +739,7 @@ void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams 
 template <int W, bool HALF>
 void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, SortedPoints &sp) {
     constexpr int NC = HALF ? 3 : 1;
-    const size_t h = (size_t)(bp.T[0] + W - 1);
-    const size_t lds = h * h * h * NC * 8;
+    const size_t lds = interp_lds_bytes(bp.T, W, NC);
     allow_lds(&k_interp_tiled<W, HALF>, lds);
     hipLaunchKernelGGL((k_interp_tiled<W, HALF>), dim3((unsigned)sp.nchunks), dim3(256), lds, cx.stream,
                        pl.fine.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2],
@@ -702,7 +748,7 @@ void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, S
 
 void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) {
     const int nc = half ? 3 : 1;
-    const BinParams bp = bin_params(pl, nc);
+    const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
     LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc, cx.stream));
     if (sp.nchunks == 0) return;
@@ -717,7 +763,7 @@ void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) 
 
 void launch_interp(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, bool half) {
     const int nc = half ? 3 : 1;
-    const BinParams bp = bin_params(pl, nc);
+    const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
     if (sp.nchunks == 0) return;
     TimedScope ts(cx, "interp");
@@ -826,6 +872,11 @@ int width_for_eps(double eps) {
     return std::max(4, std::min(12, w));
 }
 
+size_t nufft_fine_len(long long n, double eps) {
+    const size_t w = (size_t)width_for_eps(eps);
+    return next_smooth_even(std::max<size_t>(2 * (size_t)n, 2 * w));
+}
+
 NufftPlan &get_nufft_plan(Ctx &cx, const long long dims[3], double eps) {
     check_dims(dims);
     const int w = width_for_eps(eps);
@@ -841,7 +892,7 @@ NufftPlan &get_nufft_plan(Ctx &cx, const long long dims[3], double eps) {
     const double pi = 3.14159265358979323846;
     for (int a = 0; a < 3; ++a) {
         pl->n[a] = dims[a];
-        size_t f = next_smooth_even(std::max<size_t>(2 * (size_t)dims[a], 2 * (size_t)w));
+        const size_t f = nufft_fine_len(dims[a], eps);
         pl->nf[a] = (int)f;
         std::vector<float> corr(dims[a]);
         for (long long i = 0; i < dims[a]; ++i) {
@@ -855,7 +906,7 @@ NufftPlan &get_nufft_plan(Ctx &cx, const long long dims[3], double eps) {
             corr[i] = (float)(1.0 / (0.5 * w * s));
         }
         pl->corr[a].alloc(sizeof(float) * dims[a]);
-        LAM_HIP(hipMemcpy(pl->corr[a].ptr, corr.data(), sizeof(float) * dims[a], hipMemcpyHostToDevice));
+        upload_now(cx, pl->corr[a].ptr, corr.data(), sizeof(float) * dims[a]);
     }
     NufftPlan &ref = *pl;
     cx.nufft_plans.emplace(key, std::move(pl));
@@ -886,7 +937,7 @@ std::unique_ptr<Grid> make_grid(Ctx &cx, const double *alpha, const double *gamm
         g->geo_host[p] = e;
     }
     g->geo.alloc(sizeof(Geo) * P);
-    LAM_HIP(hipMemcpy(g->geo.ptr, g->geo_host.data(), sizeof(Geo) * P, hipMemcpyHostToDevice));
+    upload_now(cx, g->geo.ptr, g->geo_host.data(), sizeof(Geo) * P);
     return g;
 }
 
@@ -944,6 +995,80 @@ void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const f
     launch_spread(cx, g, pl, cbuf, true);
     fine_fft(cx, pl, 3, +1);
     crop_deapod(cx, pl, y, nullptr, 3, 1.0f / (float)(g.R * g.C), false);
+}
+
+// Times the three-component spreader and interpolator of this grid on the
+// fine grid of `dims` for a list of tile shapes (cubic, and flattened ones for
+// thin or long axes) that fit the LDS budget, and records the fastest in the
+// context's table, where every later spread/interp of the same shapes finds
+// it.  Results do not depend on the tile shape beyond FP32 summation order.
+bool bins_tuned(Ctx &cx, Grid &g, const long long dims[3], double eps) {
+    NufftPlan &pl = get_nufft_plan(cx, dims, eps);
+    std::array<int, 3> T;
+    return lookup_tuned(cx, tune_key(g, pl, 3), T);
+}
+
+std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms) {
+    NufftPlan &pl = get_nufft_plan(cx, dims, eps);
+    ensure_fine(pl, 3);
+    const size_t M = (size_t)(g.P * g.R * g.C);
+    cx.work[0].ensure(sizeof(float2) * M);
+    float2 *cbuf = cx.work[0].as<float2>();
+    LAM_HIP(hipMemsetAsync(cbuf, 0, sizeof(float2) * M, cx.stream));
+    const auto key = tune_key(g, pl, 3);
+    static const int kCands[][3] = {{6, 6, 6},  {7, 7, 7},  {8, 8, 8},  {10, 10, 10}, {8, 4, 16},
+                                    {4, 8, 16}, {8, 8, 12}, {6, 8, 12}, {12, 6, 8},   {6, 6, 12}};
+    hipEvent_t e0, e1;
+    LAM_HIP(hipEventCreate(&e0));
+    LAM_HIP(hipEventCreate(&e1));
+    std::array<int, 3> best = {0, 0, 0};
+    float best_t = 0.f;
+    const bool timing = cx.timer.on;
+    cx.timer.on = false;
+    try {
+        for (const auto &T : kCands) {
+            if (!bins_fit(T, pl, 3)) continue;
+            cx.bin_tune[key] = {T[0], T[1], T[2]};
+            launch_spread(cx, g, pl, cbuf, true);   // builds this shape's bins
+            float t = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                LAM_HIP(hipEventRecord(e0, cx.stream));
+                launch_spread(cx, g, pl, cbuf, true);
+                launch_interp(cx, g, pl, cbuf, true);
+                LAM_HIP(hipEventRecord(e1, cx.stream));
+                LAM_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                LAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                t = rep == 0 ? ms : std::min(t, ms);
+            }
+            // keep at most two shapes' bins alive: the best so far and this one
+            const std::array<int, 3> cur = {T[0], T[1], T[2]};
+            std::array<int, 3> drop = cur;
+            if (best[0] == 0 || t < best_t) {
+                drop = best;
+                best = cur;
+                best_t = t;
+            }
+            if (drop[0] != 0) g.bins.erase({pl.nf[0], pl.nf[1], pl.nf[2], pl.w, 1, drop[0], drop[1], drop[2]});
+        }
+    } catch (...) {
+        cx.timer.on = timing;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        cx.bin_tune.erase(key);
+        throw;
+    }
+    cx.timer.on = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    LAM_REQUIRE(best[0] != 0, LAM_ERR_ARG, "no spreading tile shape fits this fine grid");
+    cx.bin_tune[key] = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = best;
+    }
+    if (best_ms) *best_ms = best_t;
+    return best;
 }
 
 }  // namespace lam

@@ -21,8 +21,12 @@ struct RedTerm {
 constexpr int kMaxRedTerms = 12;
 // Computes sums of up to 12 terms over n elements; results to host (blocking).
 void reduce_terms(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_host);
-// Same, results left in a device double array (no sync).
-void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev);
+// Same, results left in a device double array (no sync).  partials (>= kRedPartials
+// doubles) replaces the context's shared scratch, for work captured into graphs.
+constexpr int kRedBlocks = 1024;
+constexpr size_t kRedPartials = (size_t)kRedBlocks * kMaxRedTerms;
+void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev,
+                      double *partials = nullptr);
 float max_value(Ctx &cx, long long n, const float *a);
 
 // ---- stencils (src/finitediff.cpp:7-71, include/bregman.h:33-36)
@@ -35,8 +39,10 @@ void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu)
 // ---- qGGMRF prior gradient (src/qggmrf.cpp:59-95): g_c += prior(x_c), c < ncomp
 void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma, float p);
 
-// ---- ramp preconditioner (include/precond.h:33-87), out = M in, on 1 or 2 volumes at once
-void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1);
+// ---- ramp preconditioner (include/precond.h:33-87), out = M in, on 1 or 2 volumes at once.
+// work (>= n complex) replaces the context's shared workspace, for work captured into graphs.
+void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1,
+                   float2 *work = nullptr);
 
 // ---- elementwise helpers
 // y = a*x + b*y over n floats (3 comps, optional)

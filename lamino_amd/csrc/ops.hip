@@ -8,8 +8,6 @@ namespace lam {
 
 namespace {
 
-constexpr int kRedBlocks = 1024;
-
 struct RedSpec {
     RedTerm t[kMaxRedTerms];
     int nterms;
@@ -234,18 +232,20 @@ unsigned vgrid(long long n) { return grid_for((size_t)n, 256, 1u << 16); }
 
 }  // namespace
 
-void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev) {
+void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev, double *partials) {
     LAM_REQUIRE(nterms >= 1 && nterms <= kMaxRedTerms, LAM_ERR_ARG, "bad reduction");
     RedSpec s;
     for (int q = 0; q < nterms; ++q) s.t[q] = terms[q];
     for (int q = nterms; q < kMaxRedTerms; ++q) s.t[q] = RedTerm{RED_SQ, nullptr, nullptr};
     s.nterms = nterms;
     s.n = n;
-    cx.red.ensure(sizeof(double) * kRedBlocks * kMaxRedTerms);
-    hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, cx.stream, s, cx.red.as<double>());
+    if (!partials) {
+        cx.red.ensure(sizeof(double) * kRedPartials);
+        partials = cx.red.as<double>();
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, cx.stream, s, partials);
     check_launch();
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, cx.stream, cx.red.as<double>(), kRedBlocks, nterms,
-                       out_dev);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, cx.stream, partials, kRedBlocks, nterms, out_dev);
     check_launch();
 }
 
@@ -305,11 +305,15 @@ void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma
     check_launch();
 }
 
-void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1) {
+void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float *in1, float *out0, float *out1,
+                   float2 *work) {
     const int n1 = (int)d[0], n2 = (int)d[1], n3 = (int)d[2];
     const long long n = d[0] * d[1] * d[2];
-    cx.work[2].ensure(sizeof(float2) * (size_t)n);
-    float2 *a = cx.work[2].as<float2>();
+    float2 *a = work;
+    if (!a) {
+        cx.work[2].ensure(sizeof(float2) * (size_t)n);
+        a = cx.work[2].as<float2>();
+    }
     // rows (n3), real or real-pair in
     {
         AxisPass p;

@@ -46,4 +46,77 @@ struct Vol3Buf {
 
 inline CVol3 cv(Vol3 v) { return CVol3{{v.c[0], v.c[1], v.c[2]}}; }
 
+// ---------------------------------------------------------------------------
+// Graph form of the solvers.  The PCG scalars (rs, pAp, alpha, beta) and its
+// stopping tests live on the device, so iterations are enqueued without any
+// host synchronisation; iterations after a stop are no-ops on the device,
+// which gives the same iterate as breaking out of the loop.  Every buffer is
+// owned by the workspace and fixed for its life, so one captured step can be
+// replayed.  Arithmetic is identical to cg_solve / split_bregman.
+
+// Device scalar block of one PCG workspace.
+struct CgScalars {
+    double sums[4];      // last reduction (3 components)
+    double rs_old, alpha, beta;
+    long long iters;     // x updates since the last begin
+    long long total;     // x updates over the workspace's life
+    int done;            // stopping test met (no further updates)
+    int pad;
+};
+
+struct FastCg {
+    Ctx *cx = nullptr;
+    long long d[3] = {0, 0, 0};
+    long long n = 0;
+    Vol3Buf r, z, p, Ap;
+    DevBuf scal;     // CgScalars
+    DevBuf red;      // reduction partials
+    DevBuf pwork;    // preconditioner workspace
+    void init(Ctx &c, const long long dims[3]);
+    CgScalars *s() const { return scal.as<CgScalars>(); }
+};
+
+// x = x0 (when distinct); r = b - A x; z = M r; p = z; rs = z.r
+void fast_cg_begin(FastCg &w, const OpFn &A, CVol3 b, CVol3 x0, Vol3 x);
+// one PCG iteration (src/conjgrad.cpp:66-107) with device-side stopping tests
+void fast_cg_iter(FastCg &w, const OpFn &A, Vol3 x, double tol);
+// blocking reads of the device scalars
+CgScalars fast_cg_read(FastCg &w);
+
+// Runs `body` (work enqueued on the context stream) eagerly the first time,
+// which creates every lazily allocated plan and buffer, then captures it once
+// into a hipGraph and replays the graph on later calls.
+class GraphStep {
+  public:
+    ~GraphStep();
+    // returns true when this call replayed the captured graph
+    bool run(Ctx &cx, bool use_graph, const std::function<void()> &body);
+
+  private:
+    int calls_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+// MBIR1 solve in graph form: x <- PCG(A, b, x0 = 0 in x); returns iterations
+long long cg_graph_solve(Ctx &cx, const long long d[3], const OpFn &A, CVol3 b, Vol3 x, long long max_iters,
+                         double tol, bool use_graph, CgStats *st);
+
+// How an MBIR job drives its solver: Plain is the host-driven loop of the
+// reference (a host read per test), Async keeps the scalars on the device,
+// Graph also replays captured iterations.  MBIR3's line search stays host-driven.
+enum class MbirMode { Plain, Async, Graph };
+
+// One reconstruction job end to end on the context's stream (capi.hip).
+void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr, float *out3,
+              lam_recon_stats *stats, MbirMode mode);
+// Upper bound of the device bytes mbir_run allocates for that job.
+size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr);
+
+// split_bregman in graph form; A must include mu * laplacian (the fused operator).
+// One outer iteration (rhs, PCG with inner_max iterations, shrink, x change) is one step.
+void split_bregman_graph(Ctx &cx, const long long d[3], const OpFn &A, CVol3 yT, CVol3 x0, double lambda, double mu,
+                         long long outer_max, long long inner_max, double tol, double xtol, Vol3 x, bool use_graph,
+                         CgStats *st, long long *outer_done);
+
 }  // namespace lam

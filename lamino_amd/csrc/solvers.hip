@@ -107,6 +107,65 @@ __global__ __launch_bounds__(256) void k_sb_shrink(CVol3 x, Vol9 d, Vol9 b, floa
     }
 }
 
+// ---- device-side PCG scalars (the graph form)
+__global__ void k_cg_begin(CgScalars *s) {
+    s->rs_old = s->sums[0] + s->sums[1] + s->sums[2];
+    s->iters = 0;
+    s->done = 0;
+}
+
+// |pAp| < 1e-10 stops before the update (src/conjgrad.cpp:73-76)
+__global__ void k_cg_alpha(CgScalars *s) {
+    if (s->done) return;
+    const double pAp = s->sums[0] + s->sums[1] + s->sums[2];
+    if (fabs(pAp) < 1e-10) {
+        s->done = 1;
+        return;
+    }
+    s->alpha = s->rs_old / pAp;
+    s->iters += 1;
+    s->total += 1;
+}
+
+// sqrt(rs_new) < tol stops after the update (src/conjgrad.cpp:94)
+__global__ void k_cg_beta(CgScalars *s, double tol) {
+    if (s->done) return;
+    const double rs_new = s->sums[0] + s->sums[1] + s->sums[2];
+    if (sqrt(rs_new) < tol) {
+        s->done = 1;
+        return;
+    }
+    s->beta = rs_new / s->rs_old;
+    s->rs_old = rs_new;
+}
+
+__global__ __launch_bounds__(256) void k_cg_step_dev(long long n, const CgScalars *s, Vol3 x, Vol3 r, CVol3 p,
+                                                     CVol3 Ap) {
+    if (s->done) return;
+    const float alpha = (float)s->alpha;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x.c[c][i] += p.c[c][i] * alpha;
+            r.c[c][i] -= Ap.c[c][i] * alpha;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_dir_dev(long long n, const CgScalars *s, CVol3 z, Vol3 p) {
+    if (s->done) return;
+    const float beta = (float)s->beta;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.c[c][i] = z.c[c][i] + p.c[c][i] * beta;
+    }
+}
+
+__global__ void k_zero_bytes(unsigned *p, long long nwords) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nwords; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 // y = x + (x - xold) * beta   (src/nagopt.cpp:156-158)
 __global__ __launch_bounds__(256) void k_nag_y(long long n, float beta, CVol3 x, CVol3 xold, Vol3 y) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -297,6 +356,185 @@ double lipschitz(Ctx &cx, const long long d[3], const OpFn &A, long long max_ite
         L = na;
     }
     return L;
+}
+
+// ------------------------------------------------------------------ graph form
+void FastCg::init(Ctx &c, const long long dims[3]) {
+    cx = &c;
+    for (int a = 0; a < 3; ++a) d[a] = dims[a];
+    n = d[0] * d[1] * d[2];
+    r.alloc(n); z.alloc(n); p.alloc(n); Ap.alloc(n);
+    scal.alloc(sizeof(CgScalars));
+    red.alloc(sizeof(double) * kRedPartials);
+    pwork.alloc(sizeof(float2) * (size_t)n);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, c.stream, scal.as<unsigned>(),
+                       (long long)(sizeof(CgScalars) / sizeof(unsigned)));
+    check_launch();
+}
+
+namespace {
+
+void fast_precond3(FastCg &w, CVol3 in, Vol3 out) {
+    float2 *wk = w.pwork.as<float2>();
+    precond_apply(*w.cx, w.d, in.c[0], in.c[1], out.c[0], out.c[1], wk);
+    precond_apply(*w.cx, w.d, in.c[2], nullptr, out.c[2], nullptr, wk);
+}
+
+void fast_dot3(FastCg &w, CVol3 a, CVol3 b) {
+    RedTerm t[3] = {{RED_DOT, a.c[0], b.c[0]}, {RED_DOT, a.c[1], b.c[1]}, {RED_DOT, a.c[2], b.c[2]}};
+    reduce_terms_dev(*w.cx, w.n, t, 3, w.s()->sums, w.red.as<double>());
+}
+
+}  // namespace
+
+void fast_cg_begin(FastCg &w, const OpFn &A, CVol3 b, CVol3 x0, Vol3 x) {
+    Ctx &cx = *w.cx;
+    if (x.c[0] != x0.c[0]) copy3(cx, w.n, x0, x);
+    A(cv(x), w.Ap.v());
+    hipLaunchKernelGGL(k_sub3, dim3(vgrid(w.n)), dim3(256), 0, cx.stream, w.n, b, w.Ap.cv(), w.r.v());
+    check_launch();
+    fast_precond3(w, w.r.cv(), w.z.v());
+    copy3(cx, w.n, w.z.cv(), w.p.v());
+    fast_dot3(w, w.z.cv(), w.r.cv());
+    hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(1), 0, cx.stream, w.s());
+    check_launch();
+}
+
+void fast_cg_iter(FastCg &w, const OpFn &A, Vol3 x, double tol) {
+    Ctx &cx = *w.cx;
+    A(w.p.cv(), w.Ap.v());
+    fast_dot3(w, w.p.cv(), w.Ap.cv());
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1), 0, cx.stream, w.s());
+    check_launch();
+    hipLaunchKernelGGL(k_cg_step_dev, dim3(vgrid(w.n)), dim3(256), 0, cx.stream, w.n, (const CgScalars *)w.s(), x,
+                       w.r.v(), w.p.cv(), w.Ap.cv());
+    check_launch();
+    fast_precond3(w, w.r.cv(), w.z.v());
+    fast_dot3(w, w.z.cv(), w.r.cv());
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1), 0, cx.stream, w.s(), tol);
+    check_launch();
+    hipLaunchKernelGGL(k_cg_dir_dev, dim3(vgrid(w.n)), dim3(256), 0, cx.stream, w.n, (const CgScalars *)w.s(),
+                       w.z.cv(), w.p.v());
+    check_launch();
+}
+
+CgScalars fast_cg_read(FastCg &w) {
+    CgScalars h;
+    LAM_HIP(hipMemcpyAsync(&h, w.scal.ptr, sizeof(CgScalars), hipMemcpyDeviceToHost, w.cx->stream));
+    LAM_HIP(hipStreamSynchronize(w.cx->stream));
+    return h;
+}
+
+GraphStep::~GraphStep() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+}
+
+bool GraphStep::run(Ctx &cx, bool use_graph, const std::function<void()> &body) {
+    ++calls_;
+    if (!use_graph || calls_ == 1) {
+        body();
+        return false;
+    }
+    if (!exec_) {
+        const bool timing = cx.timer.on;
+        cx.timer.on = false;   // no event records inside the graph
+        {
+            std::unique_lock<std::shared_mutex> lk(capture_mutex());
+            LAM_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeRelaxed));
+            capturing_here() = true;
+            try {
+                body();
+            } catch (...) {
+                capturing_here() = false;
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cx.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                cx.timer.on = timing;
+                throw;
+            }
+            capturing_here() = false;
+            LAM_HIP(hipStreamEndCapture(cx.stream, &graph_));
+        }
+        cx.timer.on = timing;
+        LAM_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+    LAM_HIP(hipGraphLaunch(exec_, cx.stream));
+    return true;
+}
+
+long long cg_graph_solve(Ctx &cx, const long long d[3], const OpFn &A, CVol3 b, Vol3 x, long long max_iters,
+                         double tol, bool use_graph, CgStats *st) {
+    FastCg w;
+    w.init(cx, d);
+    fast_cg_begin(w, A, b, cv(x), x);
+    GraphStep step;
+    constexpr long long kCheck = 8;   // iterations between host reads of the stop flag when tol > 0
+    for (long long it = 0; it < max_iters; ++it) {
+        step.run(cx, use_graph, [&] { fast_cg_iter(w, A, x, tol); });
+        if (tol > 0 && (it + 1) % kCheck == 0 && fast_cg_read(w).done) break;
+    }
+    const CgScalars s = fast_cg_read(w);
+    if (st) {
+        st->iters += s.total;
+        st->applies += 1 + max_iters;
+    }
+    return s.total;
+}
+
+void split_bregman_graph(Ctx &cx, const long long d[3], const OpFn &A, CVol3 yT, CVol3 x0, double lambda, double mu,
+                         long long outer_max, long long inner_max, double tol, double xtol, Vol3 x, bool use_graph,
+                         CgStats *st, long long *outer_done) {
+    const long long n = d[0] * d[1] * d[2];
+    FastCg w;
+    w.init(cx, d);
+    Vol3Buf xold, rhs;
+    xold.alloc(n); rhs.alloc(n);
+    DevBuf dd(sizeof(float) * 9 * (size_t)n), bb(sizeof(float) * 9 * (size_t)n), xs(sizeof(double) * 8);
+    for (DevBuf *z : {&dd, &bb}) fill(cx, 9 * n, z->as<float>(), 0.f);
+    Vol9 D, B;
+    for (int q = 0; q < 9; ++q) { D.c[q] = dd.as<float>() + q * n; B.c[q] = bb.as<float>() + q * n; }
+    copy3(cx, n, x0, xold.v());
+    if (x.c[0] != x0.c[0]) copy3(cx, n, x0, x);
+    const float eps = 1e-8f;
+    const float thr = (float)lambda / (float)mu;
+    RedTerm t[6];
+    for (int c = 0; c < 3; ++c) {
+        t[c] = RedTerm{RED_DIFF2, x.c[c], xold.v().c[c]};
+        t[3 + c] = RedTerm{RED_SQ, xold.v().c[c], nullptr};
+    }
+    // one outer iteration of src/bregman.cpp:75-131
+    auto outer_step = [&] {
+        hipLaunchKernelGGL(k_sb_rhs, dim3(vgrid(n)), dim3(256), 0, cx.stream, yT, D, B, rhs.v(), (float)mu, (int)d[0],
+                           (int)d[1], (int)d[2]);
+        check_launch();
+        fast_cg_begin(w, A, rhs.cv(), xold.cv(), x);
+        for (long long k = 0; k < inner_max; ++k) fast_cg_iter(w, A, x, tol);
+        hipLaunchKernelGGL(k_sb_shrink, dim3(vgrid(n)), dim3(256), 0, cx.stream, cv(x), D, B, eps, thr, (int)d[0],
+                           (int)d[1], (int)d[2]);
+        check_launch();
+        reduce_terms_dev(cx, n, t, 6, xs.as<double>(), w.red.as<double>());
+        copy3(cx, n, cv(x), xold.v());
+    };
+    GraphStep step;
+    long long it = 0;
+    for (; it < outer_max; ++it) {
+        step.run(cx, use_graph, outer_step);
+        if (xtol > 0) {   // the x-change test needs the host (src/bregman.cpp:122-131)
+            double s[6];
+            LAM_HIP(hipMemcpyAsync(s, xs.ptr, sizeof(s), hipMemcpyDeviceToHost, cx.stream));
+            LAM_HIP(hipStreamSynchronize(cx.stream));
+            double nd = 0;
+            for (int c = 0; c < 3; ++c) nd += std::sqrt(s[c]) / (std::sqrt(s[3 + c]) + 1e-8);
+            if (nd < xtol) { ++it; break; }
+        }
+    }
+    const CgScalars s = fast_cg_read(w);
+    if (st) {
+        st->iters += s.total;
+        st->applies += it * (1 + inner_max);
+    }
+    if (outer_done) *outer_done = it;
 }
 
 }  // namespace lam

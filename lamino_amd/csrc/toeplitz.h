@@ -25,6 +25,9 @@ struct Toep {
     long long applies = 0;
 };
 
+// L-grid length of one axis of n voxels: the smallest 2,3,5-smooth L >= 2n - 1
+int toep_len(long long n);
+
 std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], bool per_grid);
 // y = sum_g sysmat_g(x) + mu * laplacian(x)
 void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu);

@@ -237,6 +237,8 @@ void inverse_spectra(Toep &t, Vol3 y) {
 
 }  // namespace
 
+int toep_len(long long n) { return (int)next_smooth(std::max<size_t>(1, 2 * (size_t)n - 1)); }
+
 std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], bool per_grid) {
     LAM_REQUIRE(ngrids >= 1, LAM_ERR_ARG, "Toeplitz operator needs at least one grid");
     LAM_REQUIRE(!per_grid || ngrids <= kMaxQuadGrids, LAM_ERR_ARG, "too many grids for per-grid kernels");
@@ -247,7 +249,7 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
     t->per_grid = per_grid;
     for (int a = 0; a < 3; ++a) {
         t->n[a] = dims[a];
-        t->L[a] = (int)next_smooth(std::max<size_t>(1, 2 * (size_t)dims[a] - 1));
+        t->L[a] = toep_len(dims[a]);
     }
     const int L1 = t->L[0], L2 = t->L[1], L3 = t->L[2];
     const long long Lt = (long long)L1 * L2 * L3;

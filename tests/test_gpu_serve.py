@@ -1,0 +1,156 @@
+"""Batched / served reconstruction jobs (lam_server_*) on the GPU.
+
+- graph and async modes give the plain lam_mbir result (same kernels and
+  reduction order; only the setup backprojection's atomic summation order
+  may differ), and every result is within 1e-3 of the FP64 oracle;
+- a job gives the same result alone and in a mix of concurrent jobs
+  (relative difference <= 1e-5);
+- admission keeps the sum of admitted footprints within the HBM budget and
+  each job's measured device peak within its estimate;
+- a failing job reports its own error and does not stop the server.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel, say
+
+pytestmark = pytest.mark.gpu
+
+REC_TOL = 1e-3
+MIX_TOL = 1e-5
+
+
+def _job(seed, rd=(9, 5, 11), nds=1, P=8, det=9):
+    rng = np.random.default_rng(seed)
+    ds = []
+    for g in range(nds):
+        proj = rng.standard_normal((P, det, det)).astype(np.float32) + 3.0
+        alpha = np.full(P, np.radians(30.0 + 4 * g))
+        gamma = np.radians(45.0 * g) + 2 * np.pi * np.arange(P) / P
+        ds.append((proj, alpha, gamma))
+    return ds, rd
+
+
+def _params(method):
+    import lamino_amd as la
+
+    if method == 1:
+        return la.ReconParams(max_iters=6, tol=0.0)
+    if method == 2:
+        return la.ReconParams(max_iters=3, inner_iters=2, lam=0.1, mu=10.0, tol=0.0, xtol=0.0)
+    return la.ReconParams(max_iters=3, inner_iters=3, sigma=1000.0, p=1.2, mu=10.0, tol=0.0, xtol=0.0)
+
+
+@pytest.mark.parametrize("mode", ["graph", "async"])
+@pytest.mark.parametrize("method", [1, 2, 3])
+def test_server_modes_match_plain_and_oracle(ctx, orc, mode, method):
+    import lamino_amd as la
+    from lamino_amd.serve import Server
+
+    ds, rd = _job(10 + method, nds=2 if method > 1 else 1)
+    prm = _params(method)
+    plain = la.api._mbir(method, ds, rd, prm)
+    with Server(slots=2, mode=mode) as s:
+        jid = s.submit(method, ds, rd, prm)
+        x, info = s.wait(jid)
+    say(f"method {method} {mode}: vs plain {rel(x, plain):.2e}, info {info['stats']}")
+    assert info["status"] == 0
+    assert rel(x, plain) < MIX_TOL
+    kw = dict(max_iters=prm.max_iters, inner_iters=prm.inner_iters, lam=prm.lam, mu=prm.mu, sigma=prm.sigma,
+              p=prm.p, tol=prm.tol, xtol=prm.xtol)
+    xo = orc.mbir(method, ds, rd, **kw)
+    assert rel(x, xo) < REC_TOL
+
+
+def test_result_same_alone_and_in_mix(ctx):
+    from lamino_amd.serve import Server
+
+    jobs = [(2, *_job(20, rd=(9, 5, 11), nds=2)), (1, *_job(21, rd=(13, 7, 13))), (3, *_job(22, nds=2)),
+            (2, *_job(23, rd=(11, 5, 9), nds=1)), (1, *_job(24))]
+    target = 0
+    m, ds, rd = jobs[target]
+    with Server(slots=1, mode="graph") as s:
+        alone, _ = s.wait(s.submit(m, ds, rd, _params(m)))
+    with Server(slots=4, mode="graph") as s:
+        ids = [s.submit(mm, d, r, _params(mm)) for mm, d, r in jobs]
+        res = [s.wait(i) for i in ids]
+        st = s.stats()
+    starts = [r[1]["start_s"] for r in res]
+    ends = [r[1]["end_s"] for r in res]
+    overlap = any(starts[i] < ends[j] and starts[j] < ends[i] for i in range(len(res)) for j in range(i))
+    say(f"mix: stats {st}, overlapping jobs: {overlap}")
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert rel(res[target][0], alone) < MIX_TOL
+    for (mm, d, r), (x, info) in zip(jobs, res):
+        assert np.isfinite(x).all()
+        assert info["footprint_peak"] <= info["footprint_est"], info
+
+
+def test_budget_admission_serialises_jobs_that_do_not_fit(ctx):
+    import lamino_amd as la
+    from lamino_amd.serve import Server
+
+    jobs = [_job(30 + k, rd=(11, 5, 11)) for k in range(3)]
+    prm = _params(1)
+    est = max(la.api.mbir_footprint(1, ds, rd, prm) for ds, rd in jobs)
+    budget = int(est * 1.5)   # room for one job at a time
+    with Server(budget_bytes=budget, slots=3, mode="graph") as s:
+        ids = [s.submit(1, ds, rd, prm) for ds, rd in jobs]
+        infos = [s.wait(i)[1] for i in ids]
+        st = s.stats()
+    say("budget", budget, "peak admitted", st["peak_in_use"])
+    assert st["peak_in_use"] <= budget
+    spans = sorted((i["start_s"], i["end_s"]) for i in infos)
+    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+        assert b0 >= a1, spans
+
+
+def test_over_budget_job_is_rejected(ctx):
+    from lamino_amd import _lib
+    from lamino_amd.serve import Server
+
+    ds, rd = _job(40)
+    with Server(budget_bytes=1 << 20, slots=1, mode="graph") as s:
+        with pytest.raises(_lib.LaminoError) as e:
+            s.submit(1, ds, rd, _params(1))
+    assert e.value.code == 9   # LAM_ERR_BUDGET
+
+
+def test_failing_job_reports_its_error_and_server_continues(ctx):
+    from lamino_amd import _lib
+    from lamino_amd.serve import Server
+
+    ds, rd = _job(50)
+    bad = [(np.zeros_like(ds[0][0]), ds[0][1], ds[0][2])]   # max(proj) = 0: division by zero
+    with Server(slots=2, mode="graph") as s:
+        j_bad = s.submit(1, bad, rd, _params(1))
+        j_ok = s.submit(1, ds, rd, _params(1))
+        with pytest.raises(_lib.LaminoError) as e:
+            s.wait(j_bad)
+        x, info = s.wait(j_ok)
+        st = s.stats()
+    assert e.value.code == 7   # LAM_ERR_DIVZERO
+    assert info["status"] == 0 and np.isfinite(x).all()
+    assert st["failed"] == 1 and st["completed"] == 1
+
+
+def test_autotuned_tiles_keep_projector_parity(ctx, orc):
+    """The tile shape the autotuner picks changes only FP32 summation order."""
+    import lamino_amd as la
+
+    dims, R, C, P = (21, 9, 23), 31, 33, 40
+    rng = np.random.default_rng(61)
+    al = np.full(P, np.radians(35.0))
+    ga = 2 * np.pi * np.arange(P) / P
+    pg = la.PolarGrid.per_projection(al, ga, R, C)
+    proj = rng.standard_normal((P, R, C)).astype(np.float32)
+    m = rng.standard_normal((3,) + dims).astype(np.float32)
+    a0 = la.adjoint(proj, pg, dims)
+    y0 = la.forward(m, pg)
+    T, ms = la.autotune_bins(pg, dims)
+    say("autotuned tiles", T, f"{ms:.3f} ms")
+    assert all(t >= 2 for t in T) and ms > 0
+    a1 = la.adjoint(proj, pg, dims)
+    y1 = la.forward(m, pg)
+    assert rel(a1, a0) < 1e-5 and rel(y1, y0) < 1e-5
+    assert rel(a1, orc.adjoint(proj, dims, al, ga)) < 1e-4

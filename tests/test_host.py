@@ -119,3 +119,37 @@ def test_dropin_test_binary_links_and_fails_loudly_without_gpu():
     r = subprocess.run([exe, "grid_coords"], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "exception" in r.stdout and "FAIL" in r.stdout, r.stdout
+
+
+def _ds(P, det, seed=0):
+    rng = np.random.default_rng(seed)
+    return [(rng.standard_normal((P, det, det)).astype(np.float32), np.full(P, 0.5), 0.0)]
+
+
+def test_mbir_footprint_estimate_is_host_only_and_monotone():
+    """lam_mbir_footprint needs no device and grows with every shape it depends on."""
+    import lamino_amd as la
+
+    prm = la.ReconParams()
+    base = la.api.mbir_footprint(2, _ds(8, 9), (9, 5, 11), prm)
+    assert base > 64 << 20   # includes the fixed slack for plans and reductions
+    assert la.api.mbir_footprint(2, _ds(16, 9), (9, 5, 11), prm) > base          # more projections
+    assert la.api.mbir_footprint(2, _ds(8, 15), (9, 5, 11), prm) > base          # larger detector
+    assert la.api.mbir_footprint(2, _ds(8, 9), (17, 5, 11), prm) > base          # larger volume
+    assert la.api.mbir_footprint(2, _ds(8, 9) * 2, (9, 5, 11), prm) > base       # more datasets
+    # MBIR3 keeps one Toeplitz spectrum set per dataset: a second dataset costs it more than MBIR2
+    d2 = la.api.mbir_footprint(2, _ds(8, 9) * 2, (9, 5, 11), prm) - base
+    d3 = la.api.mbir_footprint(3, _ds(8, 9) * 2, (9, 5, 11), prm) - la.api.mbir_footprint(3, _ds(8, 9), (9, 5, 11), prm)
+    assert d3 > d2 > 0
+
+
+def test_server_creation_fails_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from lamino_amd import _lib
+    from lamino_amd.serve import Server
+
+    with pytest.raises(_lib.LaminoError):
+        Server(slots=2)
