@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-recon", action="store_true", help="skip the full reconstruction latency")
     ap.add_argument("--cpu-projections", type=int, default=24)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (16 uneven jobs) measurement")
+    ap.add_argument("--no-served", action="store_true", help="skip the served (Poisson arrivals) measurement")
+    ap.add_argument("--batched-jobs", type=int, default=16)
+    ap.add_argument("--served-jobs", type=int, default=12)
+    ap.add_argument("--slots", type=int, default=4, help="concurrent jobs of the batched/served server")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed fwd+adj steps plus a 1 GiB device-to-device copy (PMC calibration), "
                          "for rocprofv3 runs")
@@ -58,6 +63,94 @@ def hbm_traffic_from_profiles(kernel: str):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def _simulate_jobs(la, synth, specs):
+    """Measured projections of every job (GPU forward projector, src/forward.cpp recipe)."""
+    def project(mp, scan, dp):
+        pg = la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp)
+        return la.forward(mp, pg)
+
+    jobs = []
+    for sp in specs:
+        ds, _ = synth.simulate(sp.work, project)
+        jobs.append((sp.work.method, ds, sp.work.recon_dims, synth.recon_params(sp.work), sp.arrival_s))
+    return jobs
+
+
+def _rel(a, b):
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def run_batched(la, synth, njobs, slots):
+    """Reconstructions/hour of a mix of uneven jobs: the plain form (one job at a
+    time, lam_mbir) against the batched form (all jobs submitted at once to a
+    server: concurrent slots, hipGraph-replayed solver iterations, autotuned
+    spreader tiles).  Results of the two forms are compared job by job."""
+    from lamino_amd.serve import Server
+
+    specs = synth.batched_specs(njobs)
+    log(f"batched: simulating {len(specs)} jobs")
+    jobs = _simulate_jobs(la, synth, specs)
+    log("batched: plain form, one job at a time")
+    t0 = time.perf_counter()
+    plain = [la.api._mbir(m, ds, rd, prm) for m, ds, rd, prm, _ in jobs]
+    t_plain = time.perf_counter() - t0
+    log(f"batched: plain {t_plain:.2f} s; batched form with {slots} slots")
+    with Server(slots=slots, mode="graph") as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(m, ds, rd, prm) for m, ds, rd, prm, _ in jobs]
+        res = [srv.wait(i) for i in ids]
+        t_batch = time.perf_counter() - t0
+        st = srv.stats()
+    diffs = [_rel(x, p) for (x, _), p in zip(res, plain)]
+    for (m, ds, rd, prm, _), (x, info) in zip(jobs, res):
+        log(f"  job MBIR{m} vol {rd} x {len(ds)}x{ds[0][0].shape}: slot {info['slot']} "
+            f"start {info['start_s']:.2f} end {info['end_s']:.2f} setup {info['stats']['setup_s']:.3f} "
+            f"solve {info['stats']['solve_s']:.3f}")
+    out = {"jobs": len(jobs), "slots": slots, "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
+           "recon_per_hour_plain": round(3600 * len(jobs) / t_plain, 1),
+           "recon_per_hour_batched": round(3600 * len(jobs) / t_batch, 1),
+           "speedup": round(t_plain / t_batch, 3), "max_rel_diff_vs_plain": max(diffs),
+           "peak_admitted_bytes": st["peak_in_use"],
+           "mix": "16 jobs: volumes {[16,64,64],[32,128,128],[51,256,256]}x3, 1-3 gammas of 90-360 angles, "
+                  "tilt 25-40 deg, MBIR1 (20 CG) or MBIR2 (30x3), seeded"}
+    log("batched:", out)
+    return out
+
+
+def run_served(la, synth, njobs, slots):
+    """Continuous batching under Poisson arrivals: jobs are submitted at their
+    arrival times while others run; latency = arrival -> result on host."""
+    from lamino_amd.serve import Server
+
+    specs = synth.served_specs(njobs)
+    log(f"served: simulating {len(specs)} jobs")
+    jobs = _simulate_jobs(la, synth, specs)
+    log(f"served: {len(jobs)} Poisson arrivals (mean 4 jobs/s) on {slots} slots")
+    with Server(slots=slots, mode="graph") as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for m, ds, rd, prm, arrival in jobs:
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(m, ds, rd, prm))
+        res = [srv.wait(i) for i in ids]
+        t_all = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([info["latency_s"] for _, info in res])
+    nonfinite = [f"MBIR{m} vol {rd}" for (m, ds, rd, prm, _), (x, _) in zip(jobs, res) if not np.isfinite(x).all()]
+    waits = np.array([info["start_s"] - info["submit_s"] for _, info in res])
+    out = {"jobs": len(jobs), "slots": slots, "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_queue_wait_s": round(float(waits.mean()), 3),
+           "throughput_jobs_per_s": round(len(jobs) / t_all, 4), "makespan_s": round(t_all, 3),
+           "nonfinite_jobs": nonfinite, "peak_admitted_bytes": st["peak_in_use"],
+           "mix": "Poisson arrivals 4 jobs/s, volumes up to [51,511,511]x3, 2-4 gammas x 360 angles, "
+                  "split-Bregman or qGGMRF 50/50, seeded"}
+    log("served:", out)
+    return out
 
 
 def main():
@@ -199,8 +292,26 @@ def main():
         tr = time.perf_counter() - tr0
         recon = {"recon_s": round(tr, 4), "setup_s": round(st["setup_s"], 4), "solve_s": round(st["solve_s"], 4),
                  "normal_ops": st["normal_ops"], "cg_iters": st["cg_iters"], "outer_iters": st["outer_iters"],
+                 "cg_iters_per_s": round(st["cg_iters"] / st["solve_s"], 2),
                  "finite": bool(np.isfinite(rec).all())}
+        # the same job in graph form (device scalars, replayed hipGraph iterations); the first
+        # submission creates the worker's plans and captures the graphs, the second is timed
+        from lamino_amd.serve import Server
+
+        with Server(slots=1, mode="graph") as srv:
+            srv.wait(srv.submit(2, datasets, rd, prm))
+            tg0 = time.perf_counter()
+            recg, info = srv.wait(srv.submit(2, datasets, rd, prm))
+            tg = time.perf_counter() - tg0
+        sg = info["stats"]
+        recon["graph"] = {"recon_s": round(tg, 4), "setup_s": round(sg["setup_s"], 4),
+                          "solve_s": round(sg["solve_s"], 4), "cg_iters": sg["cg_iters"],
+                          "cg_iters_per_s": round(sg["cg_iters"] / sg["solve_s"], 2),
+                          "rel_diff_vs_plain": _rel(recg, rec)}
         log("recon:", recon)
+
+    batched = None if args.no_batched else run_batched(la, synth, args.batched_jobs, args.slots)
+    served = None if args.no_served else run_served(la, synth, args.served_jobs, args.slots)
 
     # CPU baseline: the oracle's FP32 restatement of the reference path (w=7, sigma=2 ES NUFFT,
     # per-component type-1/type-2, single-threaded detector FFTs), all host cores
@@ -239,6 +350,8 @@ def main():
         "cg_iters_per_s": round(cg_rate, 2),
         "toeplitz_build_s": round(toep_build_s, 4),
         "recon": recon,
+        "batched": batched,
+        "served": served,
         "kernels_ms_per_step": {k: round(v["total_ms"] / args.steps, 4) for k, v in kernels.items()},
     }
     print(json.dumps(line), flush=True)

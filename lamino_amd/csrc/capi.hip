@@ -17,6 +17,14 @@
 namespace lam {
 
 Ctx::~Ctx() {
+    // Buffers first: those allocated stream-ordered are freed on this stream,
+    // which must still exist.
+    fft_plans.clear();
+    nufft_plans.clear();
+    for (DevBuf &w : work) w.release();
+    red.release();
+    scal.release();
+    if (stream) (void)hipStreamSynchronize(stream);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
@@ -96,9 +104,20 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
         std::vector<double> gam(D.nproj);
         for (long long p = 0; p < D.nproj; ++p) gam[p] = D.gamma ? D.gamma[p] : D.gamma_const;
         grids.push_back(make_grid(cx, D.alpha, gam.data(), D.nproj, R2, C2));
-        // the graph form picks its spreading tiles by measurement, once per shape
-        if (mode == MbirMode::Graph && !bins_tuned(cx, *grids.back(), od, eps))
-            autotune_bins(cx, *grids.back(), od, eps, nullptr);
+        // the graph form picks its spreading tiles by measurement, once per shape, on an
+        // evenly spaced subset of the projections (same planes' density, fewer of them)
+        if (mode == MbirMode::Graph && !bins_tuned(cx, *grids.back(), od, eps)) {
+            constexpr long long kTuneProjections = 64;
+            const long long Pt = std::min<long long>(D.nproj, kTuneProjections);
+            std::vector<double> ta(Pt), tg(Pt);
+            for (long long k = 0; k < Pt; ++k) {
+                const long long i = k * D.nproj / Pt;
+                ta[k] = D.alpha[i];
+                tg[k] = gam[i];
+            }
+            auto sub = make_grid(cx, ta.data(), tg.data(), Pt, R2, C2);
+            autotune_bins(cx, *sub, od, eps, nullptr);
+        }
         float *ys[3] = {yT.v().c[0], yT.v().c[1], yT.v().c[2]};
         adjoint(cx, *grids.back(), od, eps, padded.as<float>(), ys, true);
     }

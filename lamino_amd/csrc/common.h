@@ -66,22 +66,32 @@ inline bool &capturing_here() {
     return c;
 }
 
+// Stream for stream-ordered allocation on this host thread (job-server
+// workers set their own stream): hipMallocAsync / hipFreeAsync on it never
+// synchronise the device, so concurrent jobs do not stall each other.  Unset,
+// buffers use hipMalloc / hipFree.
+inline hipStream_t &alloc_stream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
     MemTally *tally = nullptr;
+    hipStream_t stream = nullptr;   // the stream it was allocated on, when stream-ordered
     DevBuf() = default;
     explicit DevBuf(size_t b) { alloc(b); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), tally(o.tally) {
-        o.ptr = nullptr; o.bytes = 0; o.tally = nullptr;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes), tally(o.tally), stream(o.stream) {
+        o.ptr = nullptr; o.bytes = 0; o.tally = nullptr; o.stream = nullptr;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
-            ptr = o.ptr; bytes = o.bytes; tally = o.tally;
-            o.ptr = nullptr; o.bytes = 0; o.tally = nullptr;
+            ptr = o.ptr; bytes = o.bytes; tally = o.tally; stream = o.stream;
+            o.ptr = nullptr; o.bytes = 0; o.tally = nullptr; o.stream = nullptr;
         }
         return *this;
     }
@@ -91,7 +101,9 @@ struct DevBuf {
         if (b) {
             if (capturing_here()) throw Error(LAM_ERR_STATE, "device allocation inside a graph capture");
             std::shared_lock<std::shared_mutex> lk(capture_mutex());
-            LAM_HIP(hipMalloc(&ptr, b));
+            stream = alloc_stream();
+            if (stream) LAM_HIP(hipMallocAsync(&ptr, b, stream));
+            else LAM_HIP(hipMalloc(&ptr, b));
             tally = mem_tally();
             tally->cur += b;
             if (tally->cur > tally->peak) tally->peak = tally->cur;
@@ -103,12 +115,14 @@ struct DevBuf {
     void release() {
         if (ptr) {
             std::shared_lock<std::shared_mutex> lk(capture_mutex());
-            (void)hipFree(ptr);
+            if (stream) (void)hipFreeAsync(ptr, stream);
+            else (void)hipFree(ptr);
             if (tally) tally->cur -= bytes;
         }
         ptr = nullptr;
         bytes = 0;
         tally = nullptr;
+        stream = nullptr;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };

@@ -78,6 +78,7 @@ void Server::worker(int slot) {
     }
     cv_done.notify_all();
     if (!cx.own_stream) return;
+    alloc_stream() = cx.stream;   // every buffer of this worker is stream-ordered
     MemTally *tally = mem_tally();
     for (;;) {
         Job *j = nullptr;
@@ -109,12 +110,14 @@ void Server::worker(int slot) {
             st = LAM_ERR_ARG;
             err = e.what();
         }
-        (void)hipStreamSynchronize(cx.stream);
-        // the next job may have other shapes: drop this job's plans and scratch
+        // drop this job's NUFFT plans and scratch (stream-ordered frees, no device
+        // synchronisation) so that the worker holds no device memory between jobs
+        // beyond its small FFT twiddle tables
         cx.nufft_plans.clear();
         for (DevBuf &w : cx.work) w.release();
         cx.red.release();
         cx.scal.release();
+        (void)hipStreamSynchronize(cx.stream);
         const size_t peak = tally->peak - base;
         {
             std::lock_guard<std::mutex> lk(mu);

@@ -185,7 +185,8 @@ def simulate(work: Workload, project) -> list:
 
     ``project(m_padded, scan, det_padded) -> (P, det_padded, det_padded)`` is the
     forward projector to use (the GPU one in benchmarks, the oracle in tests).
-    Returns a list of (proj (P, det, det) float32, alpha, gamma) datasets.
+    Returns (datasets, m): the list of (proj (P, det, det) float32, alpha, gamma)
+    and the unpadded field.
     """
     m = vortex_field(work.recon_dims, work.seed)
     pad = [fwd_padded(n) for n in work.recon_dims]
@@ -203,3 +204,72 @@ def simulate(work: Workload, project) -> list:
             p = p + (work.noise * float(np.abs(p).max()) * rng.normal_array(p.size).reshape(p.shape)).astype(np.float32)
         out.append((p.astype(np.float32), sc.alpha, sc.gamma))
     return out, m
+
+
+# ---------------------------------------------------------------- batched / served job mixes
+VOLUME_CLASSES = ([16, 64, 64], [32, 128, 128], [51, 256, 256])
+SERVED_CLASSES = (([16, 64, 64], 0.35), ([32, 128, 128], 0.35), ([51, 256, 256], 0.2), ([51, 511, 511], 0.1))
+SB_PARAMS = dict(lam=0.1, mu=10.0, max_iters=30, inner_iters=3, tol=0.0, xtol=0.0)
+CG_PARAMS = dict(max_iters=20, tol=0.0)
+QG_PARAMS = dict(sigma=1000.0, p=1.2, mu=10.0, max_iters=50, inner_iters=3, tol=0.0, xtol=0.0)
+
+
+@dataclasses.dataclass
+class JobSpec:
+    work: Workload
+    arrival_s: float = 0.0
+
+
+def batched_specs(njobs: int = 16) -> list:
+    """Batched configuration: jobs of uneven size, volumes drawn from the three
+    plan volumes, 1-3 gamma datasets of 90-360 angles, tilt 25-40 deg, CG
+    (20 iterations) or split-Bregman (30 x 3); job seed = base + 1000 + job id."""
+    rng = SplitMix64(SEED_BASE + 4, 0)
+    out = []
+    for j in range(njobs):
+        v = VOLUME_CLASSES[min(2, int(rng.uniform() * 3))]
+        G = 1 + min(2, int(rng.uniform() * 3))
+        P = 90 * (1 + min(3, int(rng.uniform() * 4)))
+        tilt = rng.uniform(25.0, 40.0)
+        method = 1 if rng.uniform() < 0.5 else 2
+        scans = [lamino_scan(P, tilt, 45.0 * g) for g in range(G)]
+        params = dict(CG_PARAMS) if method == 1 else dict(SB_PARAMS)
+        name = f"batched job {j}: vol {v}, {G} x {P} proj, tilt {tilt:.1f}, MBIR{method}"
+        out.append(JobSpec(Workload(name, method, recon_dims_internal(v), oddify(v[1]), scans,
+                                    SEED_BASE + 1000 + j, noise=0.01, params=params)))
+    return out
+
+
+def served_specs(njobs: int = 12, rate: float = 4.0) -> list:
+    """Served configuration: Poisson arrivals (mean `rate` jobs/s), volumes up to
+    [51,511,511], 2-4 gamma datasets of 360 angles, detector = volume width,
+    split-Bregman or qGGMRF 50/50; job seed = base + 2000 + job id."""
+    rng = SplitMix64(SEED_BASE + 5, 0)
+    out = []
+    t = 0.0
+    for j in range(njobs):
+        t += -math.log(max(1e-12, 1.0 - rng.uniform())) / rate
+        u = rng.uniform()
+        acc = 0.0
+        v = SERVED_CLASSES[-1][0]
+        for vol, p in SERVED_CLASSES:
+            acc += p
+            if u < acc:
+                v = vol
+                break
+        G = 2 + min(2, int(rng.uniform() * 3))
+        tilt = rng.uniform(25.0, 40.0)
+        method = 2 if rng.uniform() < 0.5 else 3
+        scans = [lamino_scan(360, tilt, 45.0 * g) for g in range(G)]
+        params = dict(SB_PARAMS) if method == 2 else dict(QG_PARAMS)
+        name = f"served job {j}: vol {v}, {G} x 360 proj, tilt {tilt:.1f}, MBIR{method}"
+        out.append(JobSpec(Workload(name, method, recon_dims_internal(v), oddify(v[1]), scans,
+                                    SEED_BASE + 2000 + j, noise=0.01, params=params), arrival_s=t))
+    return out
+
+
+def recon_params(work: Workload):
+    """ReconParams of a workload's solver settings."""
+    from .api import ReconParams
+
+    return ReconParams(**work.params)
