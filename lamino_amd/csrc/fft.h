@@ -5,8 +5,8 @@
 // input and the output, in elements of their own type).  One workgroup owns
 // L lines: consecutive i when ci > 1 (so that the loads of a row of the tile
 // are contiguous), otherwise consecutive o2.  The transform is a mixed-radix
-// Stockham autosort (radices 2,3,4,5,7,11,13) or, for lengths with a larger
-// prime factor, Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length;
+// Stockham autosort (radices 2,3,4,5) or, for lengths with another prime
+// factor, Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length;
 // both run entirely in LDS between one load and one store.
 //
 // Fused at the load:  rotation (fftshift), zero padding beyond nv, real or

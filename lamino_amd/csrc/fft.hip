@@ -2,6 +2,7 @@
 #include <cmath>
 #include <complex>
 
+#include "fastdiv.h"
 #include "fft.h"
 
 namespace lam {
@@ -9,14 +10,19 @@ namespace lam {
 namespace {
 
 constexpr int kThreads = 512;
-constexpr int kLdsComplex = 4096;  // one LDS buffer per workgroup: L * m <= 4096 complex (32 KiB)
+constexpr int kLdsComplex = 8192;  // one LDS buffer per workgroup: L * m <= 8192 complex (64 KiB, 2 per CU)
 
+// Every index split by a launch-time divisor goes through FastDiv: a plain
+// integer division costs ~30 VALU instructions and the passes do several per
+// element.
 struct KParams {
     AxisPass a;
     int n, m, blue, nrad, sign;
     int L, Li, Lo;  // lines per workgroup, L = Li * Lo
     int rad[24];
     const float2 *roots, *chirp, *bhat;
+    FastDiv fL, fN, fM, fLi, fPad;        // tile lines, length, inner length, lines per o2 row, M - N
+    FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (M / R), span Ns
 };
 
 // LDS position of element n of tile line l: n-major when the tile runs along
@@ -77,18 +83,24 @@ __device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, in
 
 template <int R, int INNER>
 __device__ __forceinline__ void stage(float2 *__restrict__ buf, int L, int M, int Ns, int sign,
-                                      const float2 *roots) {
+                                      const float2 *roots, const FastDiv &fL, const FastDiv &fNbf,
+                                      const FastDiv &fNs) {
     constexpr int MAXB = (kLdsComplex / R + kThreads - 1) / kThreads;
     const int nbf = M / R;
     const int tot = L * nbf;
     const int tstep = M / (Ns * R);
     float2 v[MAXB][R];
+    // element e of the stage is butterfly j of tile line l
+    auto split = [&](int e, int &l, int &j) {
+        if (INNER) { j = (int)fL.div((unsigned)e); l = e - j * L; }
+        else { l = (int)fNbf.div((unsigned)e); j = e - l * nbf; }
+    };
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int e = threadIdx.x + b * kThreads;
         if (e < tot) {
             int l, j;
-            if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
+            split(e, l, j);
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = buf[lidx<INNER>(l, j + t * nbf, L, M)];
         }
@@ -99,14 +111,15 @@ __device__ __forceinline__ void stage(float2 *__restrict__ buf, int L, int M, in
         const int e = threadIdx.x + b * kThreads;
         if (e < tot) {
             int l, j;
-            if (INNER) { l = e % L; j = e / L; } else { j = e % nbf; l = e / nbf; }
-            const int k = j % Ns;
+            split(e, l, j);
+            const int jq = (int)fNs.div((unsigned)j);
+            const int k = j - jq * Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], twiddle(roots, t * k * tstep, sign));
             }
             dft<R>(v[b], sign, roots, M / R);
-            const int base = (j / Ns) * Ns * R + k;
+            const int base = jq * Ns * R + k;
 #pragma unroll
             for (int t = 0; t < R; ++t) buf[lidx<INNER>(l, base + t * Ns, L, M)] = v[b][t];
         }
@@ -120,10 +133,10 @@ __device__ void stages(float2 *buf, const KParams &P, int sign) {
     for (int s = 0; s < P.nrad; ++s) {
         const int r = P.rad[s];
         switch (r) {
-            case 2: stage<2, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
-            case 3: stage<3, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
-            case 4: stage<4, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
-            default: stage<5, INNER>(buf, P.L, P.m, Ns, sign, P.roots); break;
+            case 2: stage<2, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
+            case 3: stage<3, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
+            case 4: stage<4, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
+            default: stage<5, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
         }
         Ns *= r;
     }
@@ -134,7 +147,7 @@ __device__ __forceinline__ int remap(int o, int split, int gap) { return o < spl
 // Global coordinates of tile line l; false when the line lies past the set.
 __device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, int ibase, long long &o1, int &o2,
                                           int &i) {
-    const int lo = l / P.Li, li = l - lo * P.Li;
+    const int lo = (int)P.fLi.div((unsigned)l), li = l - lo * P.Li;
     o2 = o2base + lo;
     i = ibase + li;
     if (o2 >= P.a.c2 || i >= P.a.ci) return false;
@@ -155,7 +168,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
         int l, n;
-        if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
+        if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+        else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
         long long o1;
         int o2, i;
         int mm = n + a.rot_in;
@@ -177,7 +191,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         const int pad = M - N;
         for (int e = threadIdx.x; e < L * pad; e += blockDim.x) {
             int l, n;
-            if (INNER) { l = e % L; n = N + e / L; } else { n = N + e % pad; l = e / pad; }
+            if (INNER) { const int q = (int)P.fL.div((unsigned)e); l = e - q * L; n = N + q; }
+            else { l = (int)P.fPad.div((unsigned)e); n = N + (e - l * pad); }
             buf[lidx<INNER>(l, n, L, M)] = make_float2(0.f, 0.f);
         }
     }
@@ -189,7 +204,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         stages<INNER>(buf, P, -1);
         for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
             int l, n;
-            if (INNER) { l = e % L; n = e / L; } else { n = e % M; l = e / M; }
+            if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+            else { l = (int)P.fM.div((unsigned)e); n = e - l * M; }
             const int id = lidx<INNER>(l, n, L, M);
             buf[id] = cmul(buf[id], P.bhat[n]);
         }
@@ -199,7 +215,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
         int l, n;
-        if (INNER) { l = e % L; n = e / L; } else { n = e % N; l = e / N; }
+        if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+        else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
         long long o1;
         int o2, i;
         int mm = n + a.rot_out;
@@ -344,6 +361,19 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         grid = dim3((unsigned)((a.c2 + P.Lo - 1) / P.Lo), (unsigned)a.ci, (unsigned)a.c1);
     }
     P.L = P.Li * P.Lo;
+    P.fL = FastDiv((unsigned)P.L);
+    P.fN = FastDiv((unsigned)plan.n);
+    P.fM = FastDiv((unsigned)plan.m);
+    P.fLi = FastDiv((unsigned)P.Li);
+    P.fPad = FastDiv((unsigned)std::max(1, plan.m - plan.n));
+    {
+        int Ns = 1;
+        for (int s = 0; s < plan.nrad; ++s) {
+            P.fNbf[s] = FastDiv((unsigned)(plan.m / plan.rad[s]));
+            P.fNs[s] = FastDiv((unsigned)Ns);
+            Ns *= plan.rad[s];
+        }
+    }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2);
     TimedScope ts(cx, "fft_axis");

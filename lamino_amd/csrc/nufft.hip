@@ -599,7 +599,8 @@ PointFrame frame(const Grid &g, const NufftPlan &pl) {
 
 constexpr unsigned kChunk = 2048;  // points per spreading workgroup
 
-constexpr size_t kLdsBudget = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
+constexpr size_t kLdsBudget = 80 * 1024;     // largest box any tile shape may use (2 workgroups per CU)
+constexpr size_t kLdsDefault = 53 * 1024;    // default shape: 3 spreading workgroups per CU (160 KiB LDS)
 
 size_t box_cells(const int T[3], int w) {
     return (size_t)(T[0] + w - 1) * (size_t)(T[1] + w - 1) * (size_t)(T[2] + w - 1);
@@ -655,10 +656,13 @@ bool lookup_tuned(Ctx &cx, const std::vector<long long> &key, std::array<int, 3>
 BinParams bin_params(Ctx &cx, const Grid &g, const NufftPlan &pl, int nc) {
     std::array<int, 3> tuned;
     if (lookup_tuned(cx, tune_key(g, pl, nc), tuned)) return make_bins(tuned.data(), pl);
+    // Untuned: the largest cubic tile whose spreading box leaves room for three
+    // workgroups per CU (measured faster than two workgroups on bigger boxes:
+    // the spreader's LDS walk needs the extra waves to hide its latency).
     int T[3] = {2, 2, 2};
-    for (int cand : {16, 12, 10, 8, 6, 4, 2}) {
+    for (int cand : {16, 12, 10, 8, 7, 6, 5, 4, 2}) {
         const int c3[3] = {cand, cand, cand};
-        if (spread_lds_bytes(c3, pl.w, nc) <= kLdsBudget) {
+        if (spread_lds_bytes(c3, pl.w, nc) <= kLdsDefault) {
             T[0] = T[1] = T[2] = cand;
             break;
         }
@@ -1016,8 +1020,8 @@ std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], doub
     float2 *cbuf = cx.work[0].as<float2>();
     LAM_HIP(hipMemsetAsync(cbuf, 0, sizeof(float2) * M, cx.stream));
     const auto key = tune_key(g, pl, 3);
-    static const int kCands[][3] = {{6, 6, 6},  {7, 7, 7},  {8, 8, 8},  {10, 10, 10}, {8, 4, 16},
-                                    {4, 8, 16}, {8, 8, 12}, {6, 8, 12}, {12, 6, 8},   {6, 6, 12}};
+    static const int kCands[][3] = {{5, 5, 5}, {6, 6, 6}, {7, 7, 7}, {8, 8, 8}, {6, 4, 8},
+                                    {4, 6, 8}, {6, 6, 8}, {4, 8, 8}, {8, 4, 6}, {5, 6, 7}};
     hipEvent_t e0, e1;
     LAM_HIP(hipEventCreate(&e0));
     LAM_HIP(hipEventCreate(&e1));

@@ -153,3 +153,12 @@ def test_server_creation_fails_loudly_without_gpu():
 
     with pytest.raises(_lib.LaminoError):
         Server(slots=2)
+
+
+def test_fastdiv_matches_integer_division(tmp_path):
+    """lamino_amd/csrc/fastdiv.h (the FFT passes' index splits) against / and %."""
+    exe = str(tmp_path / "test_fastdiv")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "lamino_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cxx", "test_fastdiv.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
