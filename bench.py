@@ -207,6 +207,17 @@ def main():
     value = G * P / ((t1 - t0) / args.steps)
     log(f"fwd+adj: {ms_per_step:.3f} ms/step -> {value:.1f} projections/s")
     if args.profile:
+        # a few CG iterations of the Toeplitz normal operator + ramp preconditioner
+        T = la.ToeplitzOperator(grids, vol)
+        x0 = ctx.alloc((3,) + vol, np.float32)
+        x0.zero()
+        xd = ctx.alloc((3,) + vol, np.float32)
+        op = _lib.OP_FN(ctypes.cast(lib.lam_toep_op, ctypes.c_void_p).value)
+        it = ctypes.c_int64()
+        _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(outs[0].comps()), _ptrs(x0.comps()), 5, 0.0,
+                                    _ptrs(xd.comps()), ctypes.byref(it)))
+        ctx.sync()
+        log(f"ran {it.value} CG iterations")
         # plain copy kernel of known size: the FETCH_SIZE / WRITE_SIZE calibration
         nbytes = 1 << 30
         src, dst = ctx.alloc((nbytes // 4,), np.float32), ctx.alloc((nbytes // 4,), np.float32)
@@ -251,6 +262,20 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic_from_profiles(dom), "kernel": dom,
                 "bytes_per_launch": bytes_alg, "avg_launch_ms": round(kernels[dom]["avg_ms"], 4)}
+    # The roof that binds the NUFFT kernels is LDS, not HBM: per half-plane point the
+    # spreader read-modify-writes w^3 cells x 3 complex components (2 x 24 B each) in
+    # its LDS box and the interpolator reads them once.  Peak = 256 CUs x 128 B/clk
+    # x 2.4 GHz (an estimate from the CU's LDS width, not a published figure).
+    w = 6
+    half_points = G * P * ((det * det + 1) // 2) // G   # per launch: one dataset
+    lds_peak_tbs = 256 * 128 * 2.4e9 / 1e12
+    binding = {}
+    for k, per_pt in (("spread", 2 * 24 * w ** 3), ("interp", 24 * w ** 3)):
+        if k in kernels:
+            tbs = half_points * per_pt / (kernels[k]["avg_ms"] / 1e3) / 1e12
+            binding[k] = {"roof": "lds", "achieved_tbs": round(tbs, 2), "peak_tbs": round(lds_peak_tbs, 1),
+                          "frac": round(tbs / lds_peak_tbs, 3), "bytes_per_launch": half_points * per_pt}
+    roofline["binding"] = binding
 
     # CG iterations/s: Toeplitz normal operator + ramp preconditioner at the same shape
     log("building Toeplitz normal operator")
