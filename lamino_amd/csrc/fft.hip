@@ -10,7 +10,11 @@ namespace lam {
 namespace {
 
 constexpr int kThreads = 512;
-constexpr int kLdsComplex = 8192;  // one LDS buffer per workgroup: L * m <= 8192 complex (64 KiB, 2 per CU)
+// The workgroup's data tile: L * m <= 3840 complex (30 KiB) plus its line-base
+// tables (2 x L x 8 B <= 2 KiB).  Measured: small tiles with several workgroups
+// per CU hide the load/stage/store latency better than fewer, longer tiles.
+constexpr int kLdsComplex = 3840;
+constexpr int kMaxLines = 128;
 
 // Every index split by a launch-time divisor goes through FastDiv: a plain
 // integer division costs ~30 VALU instructions and the passes do several per
@@ -23,6 +27,7 @@ struct KParams {
     const float2 *roots, *chirp, *bhat;
     FastDiv fL, fN, fM, fLi, fPad;        // tile lines, length, inner length, lines per o2 row, M - N
     FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (M / R), span Ns
+    int isn, osn;                         // element strides along the transform (fit 32 bits)
 };
 
 // LDS position of element n of tile line l: n-major when the tile runs along
@@ -166,17 +171,28 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     const int nv = a.nv < 0 ? N : a.nv;
     const int ns = a.ns < 0 ? N : a.ns;
 
+    // 64-bit offsets of the tile's lines, once per workgroup (-1: line past the set)
+    long long *base_in = reinterpret_cast<long long *>(buf + L * M);
+    long long *base_out = base_in + L;
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        long long o1;
+        int o2, i;
+        const bool ok = tile_line(P, l, o2base, ibase, o1, o2, i);
+        base_in[l] = ok ? o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi : -1;
+        base_out[l] = ok ? o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi : -1;
+    }
+    __syncthreads();
+
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
         int l, n;
         if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
         else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
-        long long o1;
-        int o2, i;
         int mm = n + a.rot_in;
         if (mm >= N) mm -= N;
         float2 v = make_float2(0.f, 0.f);
-        if (tile_line(P, l, o2base, ibase, o1, o2, i) && (mm < nv || mm >= N - a.nv2)) {
-            const long long off = o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi + (long long)mm * a.isn;
+        const long long lb = base_in[l];
+        if (lb >= 0 && (mm < nv || mm >= N - a.nv2)) {
+            const long long off = lb + (long long)mm * P.isn;
             if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
             else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
             else {
@@ -217,16 +233,15 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         int l, n;
         if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
         else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
-        long long o1;
-        int o2, i;
         int mm = n + a.rot_out;
         if (mm >= N) mm -= N;
-        if (tile_line(P, l, o2base, ibase, o1, o2, i) && (mm < ns || mm >= N - a.ns2)) {
+        const long long lb = base_out[l];
+        if (lb >= 0 && (mm < ns || mm >= N - a.ns2)) {
             float2 v = buf[lidx<INNER>(l, n, L, M)];
             if (P.blue) v = cmul(v, P.chirp[n]);
             v.x *= a.scale;
             v.y *= a.scale;
-            const long long off = o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi + (long long)mm * a.osn;
+            const long long off = lb + (long long)mm * P.osn;
             if (a.st_mode == ST_COMPLEX) static_cast<float2 *>(a.out)[off] = v;
             else if (a.st_mode == ST_REAL) static_cast<float *>(a.out)[off] = v.x;
             else if (a.st_mode == ST_IMAG) static_cast<float *>(a.out)[off] = v.y;
@@ -349,7 +364,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.chirp = plan.chirp.as<float2>();
     P.bhat = plan.bhat.as<float2>();
     const bool inner = a.ci > 1;
-    const int cap = std::max(1, kLdsComplex / plan.m);
+    const int cap = std::max(1, std::min(kLdsComplex / plan.m, kMaxLines));
     dim3 grid;
     if (inner) {
         P.Li = std::min({a.ci, cap, 16});
@@ -361,6 +376,10 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         grid = dim3((unsigned)((a.c2 + P.Lo - 1) / P.Lo), (unsigned)a.ci, (unsigned)a.c1);
     }
     P.L = P.Li * P.Lo;
+    LAM_REQUIRE(a.isn > -(1ll << 31) && a.isn < (1ll << 31) && a.osn > -(1ll << 31) && a.osn < (1ll << 31),
+                LAM_ERR_ARG, "FFT element stride exceeds 32 bits");
+    P.isn = (int)a.isn;
+    P.osn = (int)a.osn;
     P.fL = FastDiv((unsigned)P.L);
     P.fN = FastDiv((unsigned)plan.n);
     P.fM = FastDiv((unsigned)plan.m);
@@ -375,7 +394,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         }
     }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
-    const size_t lds = (size_t)P.L * plan.m * sizeof(float2);
+    const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L;
     TimedScope ts(cx, "fft_axis");
     if (inner)
         hipLaunchKernelGGL(k_fft_axis<1>, grid, dim3(kThreads), lds, cx.stream, P);
