@@ -234,7 +234,8 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
         raw = std::max(raw, 4 * P * (size_t)D.nrows * (size_t)D.ncols);
         padded = std::max(padded, 4 * M);
         spectra = std::max(spectra, 8 * M);
-        bintmp = std::max(bintmp, 8 * half + 8 * tiles);      // tile ids, ranks, counts, offsets
+        // sort keys (two buffers), the spare index buffer, radix-sort scratch, tile counts
+        bintmp = std::max(bintmp, 16 * half + 4 * tiles + (16ull << 20));
         // sorted ids, chunk list, geometry; twice: the autotuner holds two tile shapes at once
         bins += 2 * (4 * half + 16 * (half / 1024 + tiles)) + 128 * P;
     }

@@ -14,6 +14,8 @@
 
 #include "nufft.h"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 namespace lam {
 
 namespace {
@@ -162,15 +164,20 @@ __global__ __launch_bounds__(256) void k_crop_deapod(const float2 *__restrict__ 
 }
 
 // ------------------------------------------------------------------ point binning
-// tile of every point (by its first kernel cell), and its rank inside the tile
+// Sort key of every point: its tile (by the first kernel cell) times the
+// tile's cell count plus the first cell's index inside the tile, so that a
+// stable sort by key groups the points tile by tile and, inside a tile, the
+// points that share a first kernel cell (and so touch the same cells).
+// count[tile] collects the tile sizes for the chunk list.
 template <int W, bool HALF>
-__global__ __launch_bounds__(256) void k_bin_count(const Geo *__restrict__ geo, long long P, int R, int C, int nfz,
-                                                   int nfy, int nfx, PointFrame fr, BinParams bp,
-                                                   unsigned *__restrict__ tileid, unsigned *__restrict__ rank,
-                                                   unsigned *__restrict__ count) {
+__global__ __launch_bounds__(256) void k_bin_keys(const Geo *__restrict__ geo, long long P, int R, int C, int nfz,
+                                                  int nfy, int nfx, PointFrame fr, BinParams bp,
+                                                  unsigned *__restrict__ key, unsigned *__restrict__ val,
+                                                  unsigned *__restrict__ count) {
     const long long S = (long long)R * C;
     const long long H = HALF ? (S + 1) / 2 : S;
     const long long total = P * H;
+    const unsigned tvol = (unsigned)(bp.T[0] * bp.T[1] * bp.T[2]);
     for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
          gid += (long long)gridDim.x * blockDim.x) {
         const long long p = gid / H;
@@ -181,43 +188,14 @@ __global__ __launch_bounds__(256) void k_bin_count(const Geo *__restrict__ geo, 
         const int ix = wrap0(first_cell(tx, nfx, W), nfx);
         const int iy = wrap0(first_cell(ty, nfy, W), nfy);
         const int iz = wrap0(first_cell(tz, nfz, W), nfz);
-        const unsigned tile =
-            (unsigned)(((iz / bp.T[0]) * bp.nt[1] + (iy / bp.T[1])) * bp.nt[2] + (ix / bp.T[2]));
-        tileid[gid] = tile;
-        rank[gid] = atomicAdd(&count[tile], 1u);
+        const int bz = iz / bp.T[0], by = iy / bp.T[1], bxx = ix / bp.T[2];
+        const unsigned tile = (unsigned)((bz * bp.nt[1] + by) * bp.nt[2] + bxx);
+        const unsigned local =
+            (unsigned)(((iz - bz * bp.T[0]) * bp.T[1] + (iy - by * bp.T[1])) * bp.T[2] + (ix - bxx * bp.T[2]));
+        key[gid] = tile * tvol + local;
+        val[gid] = (unsigned)gid;
+        atomicAdd(&count[tile], 1u);
     }
-}
-
-// exclusive prefix sum of n counts, one block of 1024 threads
-__global__ __launch_bounds__(1024) void k_scan_excl(const unsigned *__restrict__ count, unsigned *__restrict__ offset,
-                                                    long long n) {
-    __shared__ unsigned long long part[1024];
-    const long long per = (n + 1023) / 1024;
-    const long long b = threadIdx.x * per, e = b + per < n ? b + per : n;
-    unsigned long long s = 0;
-    for (long long i = b; i < e; ++i) s += count[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        unsigned long long v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
-    for (long long i = b; i < e; ++i) {
-        offset[i] = (unsigned)run;
-        run += count[i];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_bin_scatter(const unsigned *__restrict__ tileid,
-                                                     const unsigned *__restrict__ rank,
-                                                     const unsigned *__restrict__ offset, unsigned *__restrict__ ids,
-                                                     long long total) {
-    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
-         gid += (long long)gridDim.x * blockDim.x)
-        ids[offset[tileid[gid]] + rank[gid]] = (unsigned)gid;
 }
 
 // ------------------------------------------------------------------ tiled kernels
@@ -252,12 +230,14 @@ __device__ __forceinline__ long long box_to_fine(const TileBox &b, int cell, int
 // waves at once and no atomics are needed.  The box is then flushed once to
 // the fine grid with global atomics (neighbouring boxes overlap).
 //
-// The walk is latency-bound unless every LDS access of a point is in flight
-// at once: a point's staged record is laid out so that each lane fetches
-// what it needs (header, values, and both parity candidates of its z and y
-// weights) with a few aligned reads that do not depend on one another, the
-// next point's record is fetched while the current point is accumulated,
-// and all NC accumulator reads of a point are issued before its writes.
+// The walk is bound by LDS traffic.  The points of a chunk arrive sorted by
+// first kernel cell; a run of points with the same first cell covers the
+// same cells, so each wave sums the run in registers and does the box
+// read-modify-write once per run instead of once per point.  A point's staged
+// record is laid out so that each lane fetches what it needs (header, values,
+// and both parity candidates of its z and y weights) with a few aligned reads
+// that do not depend on one another, and the next point's record is fetched
+// while the current point is summed.
 constexpr int kStage = 128;   // points staged per batch
 
 // Staged point record, in 32-bit words: [0] box cell of the point's first
@@ -315,6 +295,39 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
         wyo[c] = SL::WY + 2 * ib;
         wxo[c] = SL::WX + e;
     }
+
+    // The points of a chunk come sorted by first kernel cell, so runs of points
+    // share one: a run is summed in registers and added to the box once.
+    float2 run[NCH][NC];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) run[c][q] = make_float2(0.f, 0.f);
+    int run_cell = -1;         // box cell of the run's first kernel cell (uniform), -1 before the first
+    float2 *run_acc = acc;     // the wave's first owned cell of that run
+    auto add_run = [&]() {
+        if (run_cell < 0) return;
+        float2 t[NCH][NC];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (act[c]) {
+#pragma unroll
+                for (int q = 0; q < NC; ++q) t[c][q] = run_acc[q * bx.ncell + loff[c]];
+            }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (act[c]) {
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    t[c][q].x += run[c][q].x;
+                    t[c][q].y += run[c][q].y;
+                    run_acc[q * bx.ncell + loff[c]] = t[c][q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NC; ++q) run[c][q] = make_float2(0.f, 0.f);
+        }
+    };
 
     for (unsigned base = 0; base < ch.z; base += kStage) {
         const int nb = (int)min((unsigned)kStage, ch.z - base);
@@ -405,25 +418,20 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
             const int cb = __builtin_amdgcn_readfirstlane(r.hdr.x);
             const int bits = __builtin_amdgcn_readfirstlane(r.hdr.y);
             const int zoff = (zpar ^ bits) & 1, yoff = (ypar ^ (bits >> 1)) & 1;
-            const int cell0 = cb + (zoff ? HYX : 0) + (yoff ? bx.HX : 0);
-            float2 t[NCH][NC];
+            if (cb != run_cell) {
+                add_run();
+                run_cell = cb;
+                run_acc = acc + cb + (zoff ? HYX : 0) + (yoff ? bx.HX : 0);
+            }
 #pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                if (act[c]) {
+            for (int c = 0; c < NCH; ++c) {
+                const float w = (zoff ? r.wz[c].y : r.wz[c].x) * (yoff ? r.wy[c].y : r.wy[c].x) * r.wx[c];
 #pragma unroll
-                    for (int q = 0; q < NC; ++q) t[c][q] = acc[q * bx.ncell + cell0 + loff[c]];
+                for (int q = 0; q < NC; ++q) {
+                    run[c][q].x += r.v[q].x * w;
+                    run[c][q].y += r.v[q].y * w;
                 }
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                if (act[c]) {
-                    const float w = (zoff ? r.wz[c].y : r.wz[c].x) * (yoff ? r.wy[c].y : r.wy[c].x) * r.wx[c];
-#pragma unroll
-                    for (int q = 0; q < NC; ++q) {
-                        t[c][q].x += r.v[q].x * w;
-                        t[c][q].y += r.v[q].y * w;
-                        acc[q * bx.ncell + cell0 + loff[c]] = t[c][q];
-                    }
-                }
+            }
         };
         // two records in flight: the next one is fetched before the current one is accumulated
         Rec ra, rb;
@@ -437,6 +445,7 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
             }
         }
     }
+    add_run();
     __syncthreads();
     for (int cell = threadIdx.x; cell < bx.ncell; cell += blockDim.x) {
         float *dst = reinterpret_cast<float *>(fine + box_to_fine(bx, cell, nfz, nfy, nfx) * NC);
@@ -681,33 +690,40 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
     if (it != g.bins.end()) return *it->second;
     TimedScope ts(cx, "bin_points");
     const long long S = g.R * g.C, H = half ? (S + 1) / 2 : S, total = g.P * H;
-    LAM_REQUIRE(total < (1ll << 32), LAM_ERR_ARG, "too many points in one polar grid for 32-bit bins");
+    LAM_REQUIRE(total < (1ll << 31), LAM_ERR_ARG, "too many points in one polar grid for 32-bit bins");
     const long long ntiles = (long long)bp.nt[0] * bp.nt[1] * bp.nt[2];
-    DevBuf tileid(sizeof(unsigned) * total), rank(sizeof(unsigned) * total);
-    DevBuf count(sizeof(unsigned) * ntiles), offset(sizeof(unsigned) * ntiles);
+    const long long nkeys = ntiles * bp.T[0] * bp.T[1] * bp.T[2];
+    LAM_REQUIRE(nkeys <= (1ll << 32), LAM_ERR_ARG, "fine grid too large for 32-bit bin keys");
+    int key_bits = 1;
+    while (key_bits < 32 && (1ll << key_bits) < nkeys) ++key_bits;
+    DevBuf key0(sizeof(unsigned) * total), key1(sizeof(unsigned) * total);
+    DevBuf val0(sizeof(unsigned) * total), val1(sizeof(unsigned) * total);
+    DevBuf count(sizeof(unsigned) * ntiles);
     LAM_HIP(hipMemsetAsync(count.ptr, 0, count.bytes, cx.stream));
     const PointFrame fr = frame(g, pl);
     const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
     const Geo *geo = g.geo.as<Geo>();
     if (half) {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_count<W, true>), dim3(grid), dim3(256), 0, cx.stream, geo, g.P,
+        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_keys<W, true>), dim3(grid), dim3(256), 0, cx.stream, geo, g.P,
                                                   (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
-                                                  tileid.as<unsigned>(), rank.as<unsigned>(), count.as<unsigned>()));
+                                                  key0.as<unsigned>(), val0.as<unsigned>(), count.as<unsigned>()));
     } else {
-        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_count<W, false>), dim3(grid), dim3(256), 0, cx.stream, geo,
+        LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_keys<W, false>), dim3(grid), dim3(256), 0, cx.stream, geo,
                                                   g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
-                                                  tileid.as<unsigned>(), rank.as<unsigned>(), count.as<unsigned>()));
+                                                  key0.as<unsigned>(), val0.as<unsigned>(), count.as<unsigned>()));
     }
     check_launch();
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, cx.stream, count.as<unsigned>(), offset.as<unsigned>(),
-                       ntiles);
-    check_launch();
+    // stable LSD radix sort of (key, point index): the order inside a key is the
+    // point order, so the binning and every sum over it are deterministic
+    hipcub::DoubleBuffer<unsigned> dk(key0.as<unsigned>(), key1.as<unsigned>());
+    hipcub::DoubleBuffer<unsigned> dv(val0.as<unsigned>(), val1.as<unsigned>());
+    size_t temp_bytes = 0;
+    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
+    DevBuf temp(std::max<size_t>(temp_bytes, 16));
+    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
     auto sp = std::make_unique<SortedPoints>();
-    sp->ids.alloc(sizeof(unsigned) * total);
     sp->npoints = total;
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, cx.stream, tileid.as<unsigned>(), rank.as<unsigned>(),
-                       offset.as<unsigned>(), sp->ids.as<unsigned>(), total);
-    check_launch();
+    sp->ids = std::move(dv.Current() == val0.as<unsigned>() ? val0 : val1);
     std::vector<unsigned> hc(ntiles);
     LAM_HIP(hipMemcpyAsync(hc.data(), count.ptr, sizeof(unsigned) * ntiles, hipMemcpyDeviceToHost, cx.stream));
     LAM_HIP(hipStreamSynchronize(cx.stream));
