@@ -235,7 +235,7 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
         padded = std::max(padded, 4 * M);
         spectra = std::max(spectra, 8 * M);
         // sort keys (two buffers), the spare index buffer, radix-sort scratch, tile counts
-        bintmp = std::max(bintmp, 16 * half + 4 * tiles + (16ull << 20));
+        bintmp = std::max(bintmp, 16 * half + 4 * tiles + ((size_t)16 << 20));
         // sorted ids, chunk list, geometry; twice: the autotuner holds two tile shapes at once
         bins += 2 * (4 * half + 16 * (half / 1024 + tiles)) + 128 * P;
     }
