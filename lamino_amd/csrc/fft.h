@@ -9,15 +9,16 @@
 // factor, Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length;
 // both run entirely in LDS between one load and one store.
 //
-// Fused at the load:  rotation (fftshift), zero padding beyond nv, real or
-// real-pair (re from in, im from in2) input.  Fused at the store: rotation
-// (ifftshift), truncation to ns, scaling, real or real-pair output.
+// Fused at the load:  rotation (fftshift), zero padding beyond nv, real,
+// real-pair (re from in, im from in2) or complex-pair (in + i in2) input.
+// Fused at the store: rotation (ifftshift), truncation to ns, scaling, real or
+// real-pair output.
 #pragma once
 #include "common.h"
 
 namespace lam {
 
-enum { LD_COMPLEX = 0, LD_REAL = 1, LD_REAL_PAIR = 2 };
+enum { LD_COMPLEX = 0, LD_REAL = 1, LD_REAL_PAIR = 2, LD_COMPLEX_PAIR = 3 };
 enum { ST_COMPLEX = 0, ST_REAL = 1, ST_REAL_PAIR = 2, ST_IMAG = 3 };
 
 struct FftPlan {
@@ -40,6 +41,9 @@ struct AxisPass {
     int nv = -1, ns = -1;      // -1: full length; valid loads m < nv, stored outputs m < ns
     int nv2 = 0, ns2 = 0;      // additionally valid / stored: the last nv2 / ns2 positions (wrapped ranges)
     int rot_in = 0, rot_out = 0;
+    // pair modes: lines with o1 >= c1_in2 have no second input (in2 reads as zero),
+    // lines with o1 >= c1_out2 store no second output
+    int c1_in2 = 1 << 30, c1_out2 = 1 << 30;
     float scale = 1.0f;
     // line sets with a hole: line o maps to o < split ? o : o + gap (wrapped mode ranges)
     int o1_split = 1 << 30, o1_gap = 0;
@@ -54,6 +58,17 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass p);
 // over P planes of R x C.  in_real: float input; out_real: store Re()*scale.
 void det_fft2(Ctx &cx, const void *in, bool in_real, void *out, bool out_real, int P, int R, int C, int sign,
               float scale, float2 *tmp);
+
+// The same transform for real projections, two planes per complex transform
+// (odd R and C only, where the mirrored detector point (R-1-j, C-1-k) is the
+// transform's negative frequency).  spec holds P planes of R x C.
+//   det_fft2_real_pairs:  spec plane 2m <- T(proj[2m] + i proj[2m+1]); the
+//       plane spectra are P2m(q) = (Z(q) + conj Z(-q)) / 2 and
+//       P2m+1(q) = (Z(q) - conj Z(-q)) / 2i.
+//   det_ifft2_real_pairs: proj[2m] + i proj[2m+1] <- scale T(spec[2m] + i spec[2m+1])
+//       for Hermitian plane spectra; spec's even planes are overwritten.
+void det_fft2_real_pairs(Ctx &cx, const float *proj, float2 *spec, int P, int R, int C, int sign);
+void det_ifft2_real_pairs(Ctx &cx, float2 *spec, float *proj, int P, int R, int C, int sign, float scale);
 
 // Full in-place 3-D complex FFT of [A][B][Cn][nc] (nc interleaved components).
 void fft3d(Ctx &cx, float2 *data, int A, int B, int Cn, int nc, int sign);

@@ -170,6 +170,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     const int ibase = INNER ? (int)blockIdx.x * P.Li : (int)blockIdx.y;
     const int nv = a.nv < 0 ? N : a.nv;
     const int ns = a.ns < 0 ? N : a.ns;
+    // the pair modes' second plane exists for this workgroup's o1 (uniform over the tile)
+    const int o1w = remap((int)blockIdx.z, a.o1_split, a.o1_gap);
+    const bool has2 = o1w < a.c1_in2, put2 = o1w < a.c1_out2;
 
     // 64-bit offsets of the tile's lines, once per workgroup (-1: line past the set)
     long long *base_in = reinterpret_cast<long long *>(buf + L * M);
@@ -195,9 +198,16 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
             const long long off = lb + (long long)mm * P.isn;
             if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
             else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
-            else {
+            else if (a.ld_mode == LD_REAL_PAIR) {
                 v.x = static_cast<const float *>(a.in)[off];
-                v.y = static_cast<const float *>(a.in2)[off];
+                if (has2) v.y = static_cast<const float *>(a.in2)[off];
+            } else {
+                v = static_cast<const float2 *>(a.in)[off];
+                if (has2) {
+                    const float2 b = static_cast<const float2 *>(a.in2)[off];
+                    v.x -= b.y;
+                    v.y += b.x;
+                }
             }
         }
         if (P.blue) v = cmul(v, P.chirp[n]);
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
             else if (a.st_mode == ST_IMAG) static_cast<float *>(a.out)[off] = v.y;
             else {
                 static_cast<float *>(a.out)[off] = v.x;
-                static_cast<float *>(a.out2)[off] = v.y;
+                if (put2) static_cast<float *>(a.out2)[off] = v.y;
             }
         }
     }
@@ -434,6 +444,63 @@ void det_fft2(Ctx &cx, const void *in, bool in_real, void *out, bool out_real, i
         a.scale = scale;
         run_axis(cx, get_fft_plan(cx, R, sign), a);
     }
+}
+
+namespace {
+
+void det_pairs(Ctx &cx, const float *proj_in, float *proj_out, float2 *spec, int P, int R, int C, int sign,
+               float scale) {
+    LAM_REQUIRE((R & 1) && (C & 1), LAM_ERR_ARG, "paired detector transforms need odd plane sizes");
+    const bool to_real = proj_out != nullptr;
+    const long long RC = (long long)R * C;
+    const int pairs = (P + 1) / 2, full = P / 2;   // pairs with a second plane: o1 < full
+    // pass 1: along columns (contiguous), lines (pair, r); pair m's spectrum lives in plane 2m
+    {
+        AxisPass a;
+        if (to_real) {
+            a.in = spec; a.in2 = spec + RC; a.ld_mode = LD_COMPLEX_PAIR;
+        } else {
+            a.in = proj_in; a.in2 = proj_in + RC; a.ld_mode = LD_REAL_PAIR;
+        }
+        a.c1_in2 = full;
+        a.is1 = 2 * RC; a.is2 = C; a.isi = 0; a.isn = 1;
+        a.out = spec;
+        a.st_mode = ST_COMPLEX;
+        a.os1 = 2 * RC; a.os2 = C; a.osi = 0; a.osn = 1;
+        a.c1 = pairs; a.c2 = R; a.ci = 1;
+        a.rot_in = C / 2;
+        a.rot_out = C / 2;
+        run_axis(cx, get_fft_plan(cx, C, sign), a);
+    }
+    // pass 2: along rows, lines (pair, c), tiled over c
+    {
+        AxisPass a;
+        a.in = spec;
+        a.ld_mode = LD_COMPLEX;
+        a.is1 = 2 * RC; a.is2 = 0; a.isi = 1; a.isn = C;
+        if (to_real) {
+            a.out = proj_out; a.out2 = proj_out + RC; a.st_mode = ST_REAL_PAIR;
+            a.c1_out2 = full;
+        } else {
+            a.out = spec; a.st_mode = ST_COMPLEX;
+        }
+        a.os1 = 2 * RC; a.os2 = 0; a.osi = 1; a.osn = C;
+        a.c1 = pairs; a.c2 = 1; a.ci = C;
+        a.rot_in = R / 2;
+        a.rot_out = R / 2;
+        a.scale = scale;
+        run_axis(cx, get_fft_plan(cx, R, sign), a);
+    }
+}
+
+}  // namespace
+
+void det_fft2_real_pairs(Ctx &cx, const float *proj, float2 *spec, int P, int R, int C, int sign) {
+    det_pairs(cx, proj, nullptr, spec, P, R, C, sign, 1.0f);
+}
+
+void det_ifft2_real_pairs(Ctx &cx, float2 *spec, float *proj, int P, int R, int C, int sign, float scale) {
+    det_pairs(cx, nullptr, proj, spec, P, R, C, sign, scale);
 }
 
 void fft3d(Ctx &cx, float2 *data, int A, int B, int Cn, int nc, int sign) {

@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
                                                       const Geo *__restrict__ geo, long long P, int R, int C,
                                                       int nfz, int nfy, int nfx, float beta, PointFrame fr,
                                                       BinParams bp, const unsigned *__restrict__ ids,
-                                                      const uint4 *__restrict__ chunks) {
+                                                      const uint4 *__restrict__ chunks, int paired) {
     constexpr int NC = HALF ? 3 : 1;
     using SL = StageLayout<W, NC>;
     constexpr int SW = SL::SW;
@@ -338,16 +338,25 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
             const long long idx = gid - p * H;
             const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
             const Geo g = geo[p];
-            float2 v = c[p * S + idx];
             float2 vc[NC];
             if constexpr (HALF) {
+                // v = c(q) + conj c(-q) of plane p (the mirror is q's negative).  Paired
+                // planes hold Z = P2m + i P2m+1 in plane 2m: the even plane's value is
+                // Z(q) + conj Z(-q), the odd one's -i (Z(q) - conj Z(-q)); the centre
+                // point (its own mirror) takes Re Z or Im Z.
                 const long long mir = S - 1 - idx;
-                if (mir != idx) v = cadd(v, cconj(c[p * S + mir]));
+                const long long pl = paired ? (p & ~1ll) : p;
+                const float2 a = c[pl * S + idx];
+                const float2 b = mir != idx ? c[pl * S + mir] : make_float2(0.f, 0.f);
+                float2 v;
+                if (!paired) v = mir != idx ? cadd(a, cconj(b)) : a;
+                else if (!(p & 1)) v = mir != idx ? make_float2(a.x + b.x, a.y - b.y) : make_float2(a.x, 0.f);
+                else v = mir != idx ? make_float2(a.y + b.y, b.x - a.x) : make_float2(a.y, 0.f);
                 vc[0] = cscale(v, (float)g.d0);
                 vc[1] = cscale(v, (float)g.d1);
                 vc[2] = cscale(v, (float)g.d2);
             } else {
-                vc[0] = v;
+                vc[0] = c[p * S + idx];
             }
             Placed<W> pt;
             place<W>(g, fr, j, k, nfz, nfy, nfx, beta, pt);
@@ -544,8 +553,9 @@ __global__ __launch_bounds__(256) void k_interp_tiled(const float2 *__restrict__
             float2 s;
             s.x = d0 * acc[0].x + d1 * acc[1].x + d2 * acc[2].x;
             s.y = d0 * acc[0].y + d1 * acc[1].y + d2 * acc[2].y;
-            c[p * S + idx] = s;
             const long long mir = S - 1 - idx;
+            if (mir == idx) s.y = 0.f;   // q = 0: the transform of a real volume is real there
+            c[p * S + idx] = s;
             if (mir != idx) c[p * S + mir] = cconj(s);
         } else {
             c[p * S + idx] = acc[0];
@@ -747,13 +757,13 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
 }
 
 template <int W, bool HALF>
-void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams &bp, SortedPoints &sp) {
+void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams &bp, SortedPoints &sp, bool paired) {
     constexpr int NC = HALF ? 3 : 1;
     const size_t lds = spread_lds_bytes(bp.T, W, NC);
     allow_lds(&k_spread_tiled<W, HALF>, lds);
     hipLaunchKernelGGL((k_spread_tiled<W, HALF>), dim3((unsigned)sp.nchunks), dim3(256), lds, cx.stream,
                        pl.fine.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2],
-                       pl.beta, frame(g, pl), bp, sp.ids.as<unsigned>(), sp.chunks.as<uint4>());
+                       pl.beta, frame(g, pl), bp, sp.ids.as<unsigned>(), sp.chunks.as<uint4>(), paired ? 1 : 0);
 }
 
 template <int W, bool HALF>
@@ -766,7 +776,8 @@ void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, S
                        pl.beta, frame(g, pl), bp, sp.ids.as<unsigned>(), sp.chunks.as<uint4>());
 }
 
-void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) {
+// paired: c holds detector-pair spectra (det_fft2_real_pairs), three-component form only
+void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, bool paired = false) {
     const int nc = half ? 3 : 1;
     const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
@@ -774,9 +785,9 @@ void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half) 
     if (sp.nchunks == 0) return;
     TimedScope ts(cx, "spread");
     if (half) {
-        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, true>(cx, g, pl, c, bp, sp)));
+        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, true>(cx, g, pl, c, bp, sp, paired)));
     } else {
-        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, false>(cx, g, pl, c, bp, sp)));
+        LAM_WIDTH_SWITCH(pl.w, (spread_w<W, false>(cx, g, pl, c, bp, sp, false)));
     }
     check_launch();
 }
@@ -977,6 +988,10 @@ void nufft1_complex(Ctx &cx, Grid &g, const long long dims[3], double eps, const
     crop_deapod(cx, pl, nullptr, f, 1, 1.0f, false);
 }
 
+// Real projections go through the detector transform two planes at a time
+// when the mirrored detector point is the transform's negative frequency.
+bool paired_planes(const Grid &g) { return (g.R & 1) && (g.C & 1) && g.P > 1; }
+
 void forward(Ctx &cx, Grid &g, const long long dims[3], double eps, const float *const m[3], float *proj) {
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
     ensure_fine(pl, 3);
@@ -986,7 +1001,9 @@ void forward(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     deapod_pad(cx, pl, m, nullptr, 3);
     fine_fft(cx, pl, 3, -1);
     launch_interp(cx, g, pl, cbuf, true);
-    det_fft2(cx, cbuf, false, proj, true, (int)g.P, (int)g.R, (int)g.C, +1, 1.0f / (float)(g.R * g.C), cbuf);
+    const float scale = 1.0f / (float)(g.R * g.C);
+    if (paired_planes(g)) det_ifft2_real_pairs(cx, cbuf, proj, (int)g.P, (int)g.R, (int)g.C, +1, scale);
+    else det_fft2(cx, cbuf, false, proj, true, (int)g.P, (int)g.R, (int)g.C, +1, scale, cbuf);
 }
 
 void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float *proj, float *const m[3],
@@ -996,8 +1013,10 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     const size_t M = (size_t)(g.P * g.R * g.C);
     cx.work[0].ensure(sizeof(float2) * M);
     float2 *cbuf = cx.work[0].as<float2>();
-    det_fft2(cx, proj, true, cbuf, false, (int)g.P, (int)g.R, (int)g.C, -1, 1.0f, cbuf);
-    launch_spread(cx, g, pl, cbuf, true);
+    const bool paired = paired_planes(g);
+    if (paired) det_fft2_real_pairs(cx, proj, cbuf, (int)g.P, (int)g.R, (int)g.C, -1);
+    else det_fft2(cx, proj, true, cbuf, false, (int)g.P, (int)g.R, (int)g.C, -1, 1.0f, cbuf);
+    launch_spread(cx, g, pl, cbuf, true, paired);
     fine_fft(cx, pl, 3, +1);
     crop_deapod(cx, pl, m, nullptr, 3, 1.0f / (float)(g.R * g.C), accumulate);
 }
