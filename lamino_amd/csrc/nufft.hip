@@ -12,6 +12,7 @@
 #include <cmath>
 #include <mutex>
 
+#include "fastdiv.h"
 #include "nufft.h"
 
 #include <hipcub/device/device_radix_sort.hpp>
@@ -32,10 +33,12 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 // that places a point (binning, spreading, interpolation) computes bitwise the
 // same fine coordinate and first kernel cell.
 
-// Folds t into [0, nf) and returns the first kernel cell ceil(t - W/2).
-__device__ __forceinline__ int first_cell(double &t, int nf, int W) {
+// Folds t into one period and returns the first kernel cell ceil(t - W/2).
+// The fold multiplies by the reciprocal 1/nf: at an exact multiple of nf it
+// may land on -0 ulp or nf, which every caller wraps (wrap0) identically.
+__device__ __forceinline__ int first_cell(double &t, int nf, double rnf, int W) {
 #pragma clang fp contract(off)
-    const double fl = floor(t / (double)nf);
+    const double fl = floor(t * rnf);
     t -= fl * (double)nf;
     return (int)ceil(t - 0.5 * W);
 }
@@ -49,12 +52,14 @@ __device__ __forceinline__ void es_weights(double t, int i0, float beta, float *
     for (int m = 0; m < W; ++m) {
         const float z = z0 + dz * (float)m;
         const float r = 1.0f - z * z;
-        wts[m] = r > 0.f ? __expf(beta * (sqrtf(r) - 1.0f)) : 0.f;
+        // r in (0, 1]: the hardware square root (no denormal rescaling needed)
+        wts[m] = r > 0.f ? __expf(beta * (__builtin_amdgcn_sqrtf(r) - 1.0f)) : 0.f;
     }
 }
 
 struct PointFrame {
     double dX, dY, k0, j0, sx, sy, sz;  // qX step, qY step, centres, fine scales nf/(2 pi)
+    double rx, ry, rz;                  // 1 / nf per axis
 };
 
 __device__ __forceinline__ void point_coords(const Geo &g, const PointFrame &f, int j, int k, double &tx, double &ty,
@@ -79,9 +84,9 @@ __device__ __forceinline__ void place(const Geo &g, const PointFrame &fr, int j,
                                       float beta, Placed<W> &pt) {
     double tx, ty, tz;
     point_coords(g, fr, j, k, tx, ty, tz);
-    pt.ix = first_cell(tx, nfx, W);
-    pt.iy = first_cell(ty, nfy, W);
-    pt.iz = first_cell(tz, nfz, W);
+    pt.ix = first_cell(tx, nfx, fr.rx, W);
+    pt.iy = first_cell(ty, nfy, fr.ry, W);
+    pt.iz = first_cell(tz, nfz, fr.rz, W);
     es_weights<W>(tx, pt.ix, beta, pt.wx);
     es_weights<W>(ty, pt.iy, beta, pt.wy);
     es_weights<W>(tz, pt.iz, beta, pt.wz);
@@ -93,6 +98,7 @@ __device__ __forceinline__ void place(const Geo &g, const PointFrame &fr, int j,
 struct BinParams {
     int T[3];    // tile extent z, y, x
     int nt[3];   // tiles per axis
+    FastDiv fhx, fhyx;   // halo box row length HX and plane size HY * HX
 };
 
 __device__ __forceinline__ int wrap0(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -185,9 +191,9 @@ __global__ __launch_bounds__(256) void k_bin_keys(const Geo *__restrict__ geo, l
         const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
         double tx, ty, tz;
         point_coords(geo[p], fr, j, k, tx, ty, tz);
-        const int ix = wrap0(first_cell(tx, nfx, W), nfx);
-        const int iy = wrap0(first_cell(ty, nfy, W), nfy);
-        const int iz = wrap0(first_cell(tz, nfz, W), nfz);
+        const int ix = wrap0(first_cell(tx, nfx, fr.rx, W), nfx);
+        const int iy = wrap0(first_cell(ty, nfy, fr.ry, W), nfy);
+        const int iz = wrap0(first_cell(tz, nfz, fr.rz, W), nfz);
         const int bz = iz / bp.T[0], by = iy / bp.T[1], bxx = ix / bp.T[2];
         const unsigned tile = (unsigned)((bz * bp.nt[1] + by) * bp.nt[2] + bxx);
         const unsigned local =
@@ -216,9 +222,16 @@ __device__ __forceinline__ TileBox tile_box(int tile, const BinParams &bp) {
     return b;
 }
 
-__device__ __forceinline__ long long box_to_fine(const TileBox &b, int cell, int nfz, int nfy, int nfx) {
-    const int cx_ = cell % b.HX, cy_ = (cell / b.HX) % b.HY, cz_ = cell / (b.HX * b.HY);
-    const int gz = (b.oz + cz_) % nfz, gy = (b.oy + cy_) % nfy, gx = (b.ox + cx_) % nfx;
+__device__ __forceinline__ int wrap_up(int i, int n) {
+    while (i >= n) i -= n;   // a box reaches at most a few periods past its origin
+    return i;
+}
+
+__device__ __forceinline__ long long box_to_fine(const TileBox &b, const BinParams &bp, int cell, int nfz, int nfy,
+                                                 int nfx) {
+    const int cz_ = (int)bp.fhyx.div((unsigned)cell), rem = cell - cz_ * b.HY * b.HX;
+    const int cy_ = (int)bp.fhx.div((unsigned)rem), cx_ = rem - cy_ * b.HX;
+    const int gz = wrap_up(b.oz + cz_, nfz), gy = wrap_up(b.oy + cy_, nfy), gx = wrap_up(b.ox + cx_, nfx);
     return ((long long)gz * nfy + gy) * nfx + gx;
 }
 
@@ -457,7 +470,7 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
     add_run();
     __syncthreads();
     for (int cell = threadIdx.x; cell < bx.ncell; cell += blockDim.x) {
-        float *dst = reinterpret_cast<float *>(fine + box_to_fine(bx, cell, nfz, nfy, nfx) * NC);
+        float *dst = reinterpret_cast<float *>(fine + box_to_fine(bx, bp, cell, nfz, nfy, nfx) * NC);
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
             const float2 t = acc[q * bx.ncell + cell];
@@ -482,7 +495,7 @@ __global__ __launch_bounds__(256) void k_interp_tiled(const float2 *__restrict__
     float2 *box = reinterpret_cast<float2 *>(lds);   // [ncell][NC]
     for (int i = threadIdx.x; i < NC * bx.ncell; i += blockDim.x) {
         const int cell = i / NC, q = i - cell * NC;
-        box[i] = fine[box_to_fine(bx, cell, nfz, nfy, nfx) * NC + q];
+        box[i] = fine[box_to_fine(bx, bp, cell, nfz, nfy, nfx) * NC + q];
     }
     __syncthreads();
     const long long S = (long long)R * C;
@@ -603,6 +616,9 @@ PointFrame frame(const Grid &g, const NufftPlan &pl) {
     f.sx = (double)pl.nf[2] / kTwoPi;
     f.sy = (double)pl.nf[1] / kTwoPi;
     f.sz = (double)pl.nf[0] / kTwoPi;
+    f.rx = 1.0 / (double)pl.nf[2];
+    f.ry = 1.0 / (double)pl.nf[1];
+    f.rz = 1.0 / (double)pl.nf[0];
     return f;
 }
 
@@ -638,6 +654,9 @@ BinParams make_bins(const int T[3], const NufftPlan &pl) {
         bp.T[a] = T[a];
         bp.nt[a] = (pl.nf[a] + T[a] - 1) / T[a];
     }
+    const unsigned hx = (unsigned)(T[2] + pl.w - 1), hy = (unsigned)(T[1] + pl.w - 1);
+    bp.fhx = FastDiv(hx);
+    bp.fhyx = FastDiv(hx * hy);
     return bp;
 }
 
