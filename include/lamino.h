@@ -163,7 +163,10 @@ lam_status lam_toep_op(void *user, const float *const x[3], float *const y[3]);
  * bins internally).  Times candidate tile shapes (z, y, x) of this grid's
  * three-component spread + interpolation on the fine grid of dims and keeps
  * the fastest for every later call with the same fine grid and detector, in
- * this process.  tile_out: the chosen shape; best_ms: its spread + interp time. */
+ * this process.  tile_out: the chosen shape; best_ms: its spread + interp time.
+ * Grids whose projections share the tilt use the row-factorised projectors,
+ * whose tiling follows from the fine grid: tile_out is then (Tz, W, Tx), the
+ * (z, x) tile and the kernel width, and best_ms their measured time. */
 lam_status lam_autotune_bins(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, int tile_out[3],
                              double *best_ms);
 

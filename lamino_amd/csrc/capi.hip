@@ -226,7 +226,7 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
         tiles *= (f + 1) / 2;                       // bins are never smaller than 2 cells
         lt *= (size_t)toep_len(od[a]);
     }
-    size_t raw = 0, padded = 0, spectra = 0, bintmp = 0, bins = 0;
+    size_t raw = 0, padded = 0, spectra = 0, bintmp = 0, bins = 0, rows_slab = 0;
     for (int g = 0; g < nds; ++g) {
         const lam_dataset &D = ds[g];
         const size_t R2 = (size_t)pad_n(D.nrows, dpad), C2 = (size_t)pad_n(D.ncols, dpad);
@@ -238,6 +238,11 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
         bintmp = std::max(bintmp, 16 * half + 4 * tiles + ((size_t)16 << 20));
         // sorted ids, chunk list, geometry; twice: the autotuner holds two tile shapes at once
         bins += 2 * (4 * half + 16 * (half / 1024 + tiles)) + 128 * P;
+        // the row-factorised projectors (grids sharing the tilt; counted for every grid)
+        const RowBytes rb = row_path_bytes(od, eps, (long long)P, (long long)R2, (long long)C2);
+        rows_slab = std::max(rows_slab, rb.slab);
+        bintmp = std::max(bintmp, rb.setup);
+        bins += rb.kept;
     }
     const size_t fine = 24 * nf;                                   // 3 complex components
     const size_t toep = 24 * lt * (1 + (pr.method == 3 ? (size_t)nds : 0)) + 32 * lt;
@@ -247,7 +252,7 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
     if (pr.method == 3) solve += 36 * n;                           // xold y g
     const size_t crop = 4 * (size_t)(rd[0] * rd[1] * rd[2]);
     const size_t slack = 64ull << 20;                              // plans, reductions, twiddles
-    return 12 * n + raw + padded + spectra + bintmp + bins + fine + toep + solve + crop + slack;
+    return 12 * n + raw + padded + spectra + bintmp + bins + rows_slab + fine + toep + solve + crop + slack;
 }
 
 }  // namespace lam

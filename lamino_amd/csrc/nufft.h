@@ -5,6 +5,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "fastdiv.h"
 #include "fft.h"
 
 namespace lam {
@@ -29,15 +30,54 @@ struct SortedPoints {
     long long npoints = 0;
 };
 
+// Row-factorised form of the three-component projector NUFFT, for grids whose
+// projections share the tilt alpha: then v_p has the same y component for
+// every p and u_p has none, so a point's y = qY_j (-sin alpha) depends on its
+// detector row j only.  The ES kernel factorises as wy_j(y) x [wz wx](z, x):
+//   type 2: G_j(z, x) = sum_b wy_j[b] F(z, iy_j + b, x), then a 2-D gather of
+//           W^2 taps per point from the (z, x) slab of its row;
+//   type 1: a 2-D spread of W^2 taps per point into H_j(z, x), then
+//           F(z, iy_j + b, x) += wy_j[b] H_j(z, x).
+// The (z, x) plane is cut into tiles whose edges divide the fine-grid
+// lengths; one workgroup owns one (row, tile).  The spreader stores its
+// tile's interior to the slab and the parts of its box that reach into the
+// next tiles (south and east strips of W - 1 cells, and their corner) to a
+// halo buffer, and the y pass adds the halos of a cell's neighbours, so no
+// value is written twice and no atomics are needed.  Rows are handled in
+// groups whose slabs and halos fit a byte budget.
+struct RowGeom {
+    int Tz = 0, Tx = 0;        // tile edges, dividing nfz and nfx
+    int ntz = 0, ntx = 0;      // tiles per axis
+    unsigned ntiles = 0;
+    int HBz = 0, HBx = 0;      // box extents Tz + W - 1, Tx + W - 1
+    int NH = 0;                // halo cells per tile: (W-1) Tx + Tz (W-1) + (W-1)^2
+    FastDiv fbx;               // HBx
+};
+
+struct RowPlan {
+    RowGeom rg;
+    int Jn = 0;                        // rows of the half point set
+    int dir = 1;                       // +1: iy non-decreasing in j, -1: non-increasing
+    DevBuf wy;                         // float [Jn][W] row weights
+    DevBuf iy;                         // int [Jn] unwrapped first y cell of each row
+    DevBuf count;                      // unsigned [Jn][ntiles] points per (row, tile)
+    DevBuf ids;                        // half-point ids sorted by (row, tile, first cell)
+    DevBuf chunks;                     // uint4 (row * ntiles + tile, first, count, 0), one per non-empty (row, tile)
+    long long nchunks = 0;
+    std::vector<long long> row_chunk;  // first chunk of each row, then nchunks (Jn + 1 entries)
+};
+
 struct Grid {
     Ctx *cx = nullptr;
     long long P = 0, R = 0, C = 0;
     std::vector<double> alpha, gamma;
     bool literal = false;      // built by the reference ctor (constant gamma)
+    bool shared_tilt = false;  // every projection has the same alpha (row-factorised projectors)
     double gamma_const = 0;
     DevBuf geo;                // Geo[P]
     std::vector<Geo> geo_host;
     std::map<std::vector<long long>, std::unique_ptr<SortedPoints>> bins;
+    std::map<std::vector<long long>, std::unique_ptr<RowPlan>> rowplans;
     ~Grid();
 };
 
@@ -48,6 +88,8 @@ struct NufftPlan {
     float beta = 13.8f;
     DevBuf corr[3];               // float 1/psi_hat per axis, indexed by array index
     DevBuf fine;                  // float2 [nfz][nfy][nfx][3]
+    DevBuf slab;                  // float2 [rows][nfz][nfx][3] of the row-factorised path
+    DevBuf halo;                  // float2 [rows][tiles][halo cells][3] of the row-factorised spreader
 };
 
 int width_for_eps(double eps);
@@ -70,6 +112,15 @@ void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const f
 std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms);
 // whether a tuned shape exists (in this context or the process-wide table)
 bool bins_tuned(Ctx &cx, Grid &g, const long long dims[3], double eps);
+// Device bytes of the row-factorised projectors of one shared-tilt grid of P
+// projections of R x C on the fine grid of dims: slab: the row group's slabs
+// and halos (kept by the plan, shared by grids of the same dims), kept: the
+// grid's row bins, setup: temporaries while they are built.  Zero when the
+// fine grid has no row tiling.
+struct RowBytes {
+    size_t slab = 0, kept = 0, setup = 0;
+};
+RowBytes row_path_bytes(const long long dims[3], double eps, long long P, long long R, long long C);
 
 }  // namespace lam
 

@@ -10,6 +10,7 @@
 //     c(q) + conj(c(-q)) over the half set (exact, no symmetry assumed).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 
 #include "fastdiv.h"
@@ -235,6 +236,41 @@ __device__ __forceinline__ long long box_to_fine(const TileBox &b, const BinPara
     return ((long long)gz * nfy + gy) * nfx + gx;
 }
 
+// The three component values a half-set point spreads: v = c(q) + conj c(-q)
+// of plane p (the mirror is q's negative), times d_p.  Paired planes hold
+// Z = P2m + i P2m+1 in plane 2m: the even plane's value is Z(q) + conj Z(-q),
+// the odd one's -i (Z(q) - conj Z(-q)); the centre point (its own mirror)
+// takes Re Z or Im Z.
+__device__ __forceinline__ void half_point_values(const float2 *__restrict__ c, long long p, long long idx,
+                                                  long long S, const Geo &g, int paired, float2 vc[3]) {
+    const long long mir = S - 1 - idx;
+    const long long pl = paired ? (p & ~1ll) : p;
+    const float2 a = c[pl * S + idx];
+    const float2 b = mir != idx ? c[pl * S + mir] : make_float2(0.f, 0.f);
+    float2 v;
+    if (!paired) v = mir != idx ? cadd(a, cconj(b)) : a;
+    else if (!(p & 1)) v = mir != idx ? make_float2(a.x + b.x, a.y - b.y) : make_float2(a.x, 0.f);
+    else v = mir != idx ? make_float2(a.y + b.y, b.x - a.x) : make_float2(a.y, 0.f);
+    vc[0] = cscale(v, (float)g.d0);
+    vc[1] = cscale(v, (float)g.d1);
+    vc[2] = cscale(v, (float)g.d2);
+}
+
+// The interpolated value of a half-set point and its mirror: s = sum_i d_i c_i
+// at q, conj(s) at -q; at q = 0 (its own mirror) the transform of a real
+// volume is real.
+__device__ __forceinline__ void store_half_point(float2 *__restrict__ c, long long p, long long idx, long long S,
+                                                 const Geo &g, const float2 acc[3]) {
+    const float d0 = (float)g.d0, d1 = (float)g.d1, d2 = (float)g.d2;
+    float2 s;
+    s.x = d0 * acc[0].x + d1 * acc[1].x + d2 * acc[2].x;
+    s.y = d0 * acc[0].y + d1 * acc[1].y + d2 * acc[2].y;
+    const long long mir = S - 1 - idx;
+    if (mir == idx) s.y = 0.f;
+    c[p * S + idx] = s;
+    if (mir != idx) c[p * S + mir] = cconj(s);
+}
+
 // ---- spreading (type 1)
 // The box accumulator lives in LDS as NC complex planes.  The four waves of
 // the workgroup own the four (z, y) parity classes of box cells; every wave
@@ -352,25 +388,8 @@ __global__ __launch_bounds__(256) void k_spread_tiled(float2 *__restrict__ fine,
             const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
             const Geo g = geo[p];
             float2 vc[NC];
-            if constexpr (HALF) {
-                // v = c(q) + conj c(-q) of plane p (the mirror is q's negative).  Paired
-                // planes hold Z = P2m + i P2m+1 in plane 2m: the even plane's value is
-                // Z(q) + conj Z(-q), the odd one's -i (Z(q) - conj Z(-q)); the centre
-                // point (its own mirror) takes Re Z or Im Z.
-                const long long mir = S - 1 - idx;
-                const long long pl = paired ? (p & ~1ll) : p;
-                const float2 a = c[pl * S + idx];
-                const float2 b = mir != idx ? c[pl * S + mir] : make_float2(0.f, 0.f);
-                float2 v;
-                if (!paired) v = mir != idx ? cadd(a, cconj(b)) : a;
-                else if (!(p & 1)) v = mir != idx ? make_float2(a.x + b.x, a.y - b.y) : make_float2(a.x, 0.f);
-                else v = mir != idx ? make_float2(a.y + b.y, b.x - a.x) : make_float2(a.y, 0.f);
-                vc[0] = cscale(v, (float)g.d0);
-                vc[1] = cscale(v, (float)g.d1);
-                vc[2] = cscale(v, (float)g.d2);
-            } else {
-                vc[0] = c[p * S + idx];
-            }
+            if constexpr (HALF) half_point_values(c, p, idx, S, g, paired, vc);
+            else vc[0] = c[p * S + idx];
             Placed<W> pt;
             place<W>(g, fr, j, k, nfz, nfy, nfx, beta, pt);
             const int lz = wrap0(pt.iz, nfz) - bx.oz, ly = wrap0(pt.iy, nfy) - bx.oy, lx = wrap0(pt.ix, nfx) - bx.ox;
@@ -562,17 +581,421 @@ __global__ __launch_bounds__(256) void k_interp_tiled(const float2 *__restrict__
             }
         }
         if constexpr (HALF) {
-            const float d0 = (float)g.d0, d1 = (float)g.d1, d2 = (float)g.d2;
-            float2 s;
-            s.x = d0 * acc[0].x + d1 * acc[1].x + d2 * acc[2].x;
-            s.y = d0 * acc[0].y + d1 * acc[1].y + d2 * acc[2].y;
-            const long long mir = S - 1 - idx;
-            if (mir == idx) s.y = 0.f;   // q = 0: the transform of a real volume is real there
-            c[p * S + idx] = s;
-            if (mir != idx) c[p * S + mir] = cconj(s);
+            store_half_point(c, p, idx, S, g, acc);
         } else {
             c[p * S + idx] = acc[0];
         }
+    }
+}
+
+// ------------------------------------------------------------------ row-factorised path
+// (see RowPlan).  Row j of the half point set has the unwrapped first y cell
+// iy[j] and the weights wy[j][0..W); its slab holds the (z, x) plane of
+// [nfz][nfx][3] complex values.  Rows are processed in groups [j0, j0 + nrows).
+
+__device__ __forceinline__ int wrap_any(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// Sort key of a half point for the row path: (row, 2-D tile of its (z, x)
+// first kernel cell, that cell inside the tile).
+template <int W>
+__global__ __launch_bounds__(256) void k_bin_keys_rows(const Geo *__restrict__ geo, long long P, int R, int C,
+                                                       int nfz, int nfx, PointFrame fr, RowGeom rg,
+                                                       unsigned *__restrict__ key, unsigned *__restrict__ val,
+                                                       unsigned *__restrict__ count) {
+    const long long S = (long long)R * C, H = (S + 1) / 2, total = P * H;
+    const unsigned tt = (unsigned)(rg.Tz * rg.Tx);
+    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * blockDim.x) {
+        const long long p = gid / H;
+        const long long idx = gid - p * H;
+        const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
+        double tx, ty, tz;
+        point_coords(geo[p], fr, j, k, tx, ty, tz);
+        const int ix = wrap0(first_cell(tx, nfx, fr.rx, W), nfx);
+        const int iz = wrap0(first_cell(tz, nfz, fr.rz, W), nfz);
+        const int bz = iz / rg.Tz, bx = ix / rg.Tx;
+        const unsigned jt = (unsigned)j * rg.ntiles + (unsigned)(bz * rg.ntx + bx);
+        key[gid] = jt * tt + (unsigned)((iz - bz * rg.Tz) * rg.Tx + (ix - bx * rg.Tx));
+        val[gid] = (unsigned)gid;
+        atomicAdd(&count[jt], 1u);
+    }
+}
+
+// type 2, step 1: G_r(z, x) = sum_b wy[j][b] F(z, iy[j] + b, x), j = j0 + r.  One thread per
+// (z, x, component) walks the group's rows in the order of non-decreasing iy with
+// a window of W fine-grid values in registers, so each value is loaded once.
+template <int W>
+__global__ __launch_bounds__(256) void k_rows_contract(const float2 *__restrict__ fine, float2 *__restrict__ slab,
+                                                       const float *__restrict__ wy, const int *__restrict__ iy,
+                                                       int nfz, int nfy, int nfx, int j0, int nrows, int dir) {
+    const long long nxc = (long long)nfx * 3, total = (long long)nfz * nxc;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(id / nxc);
+        const long long xc = id - (long long)z * nxc;
+        const float2 *col = fine + (long long)z * nfy * nxc + xc;
+        int ybase = iy[j0 + (dir > 0 ? 0 : nrows - 1)];
+        float2 win[W];
+#pragma unroll
+        for (int b = 0; b < W; ++b) win[b] = col[(long long)wrap_any(ybase + b, nfy) * nxc];
+        for (int t = 0; t < nrows; ++t) {
+            const int r = dir > 0 ? t : nrows - 1 - t;
+            const int j = j0 + r;
+            const int iyr = iy[j];
+            while (ybase < iyr) {
+#pragma unroll
+                for (int b = 0; b < W - 1; ++b) win[b] = win[b + 1];
+                win[W - 1] = col[(long long)wrap_any(ybase + W, nfy) * nxc];
+                ++ybase;
+            }
+            const float *w = wy + (long long)j * W;
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                acc.x += w[b] * win[b].x;
+                acc.y += w[b] * win[b].y;
+            }
+            slab[((long long)r * nfz + z) * nxc + xc] = acc;
+        }
+    }
+}
+
+// type 1, step 2: F(z, iy[j] + b, x) += wy[j][b] H_r(z, x) over the group's rows.  H_r(z, x)
+// is the interior value its tile's workgroup stored plus the halo strips of the
+// tiles above, left and above-left whose boxes reach (z, x); tiles without points
+// stored nothing and count as zero.  One thread per (z, x, component) owns that
+// fine-grid column (no atomics), with a window of W accumulators in registers.
+template <int W>
+__global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, const float2 *__restrict__ slab,
+                                                     const float2 *__restrict__ halo, const unsigned *__restrict__ count,
+                                                     const float *__restrict__ wy, const int *__restrict__ iy,
+                                                     int nfz, int nfy, int nfx, RowGeom rg, int j0, int nrows,
+                                                     int dir) {
+    const long long nxc = (long long)nfx * 3, total = (long long)nfz * nxc;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(id / nxc);
+        const long long xc = id - (long long)z * nxc;
+        const int x = (int)(xc / 3), m = (int)(xc - 3ll * x);
+        const int tz = z / rg.Tz, lz = z - tz * rg.Tz, tx = x / rg.Tx, lx = x - tx * rg.Tx;
+        const int tzu = tz > 0 ? tz - 1 : rg.ntz - 1, txl = tx > 0 ? tx - 1 : rg.ntx - 1;
+        const unsigned t0 = (unsigned)(tz * rg.ntx + tx);
+        const bool south = lz < W - 1, east = lx < W - 1;
+        const unsigned tS = (unsigned)(tzu * rg.ntx + tx), tE = (unsigned)(tz * rg.ntx + txl),
+                       tC = (unsigned)(tzu * rg.ntx + txl);
+        const int hS = lz * rg.Tx + lx, hE = (W - 1) * rg.Tx + lz * (W - 1) + lx,
+                  hC = (W - 1) * rg.Tx + rg.Tz * (W - 1) + lz * (W - 1) + lx;
+        float2 *col = fine + (long long)z * nfy * nxc + xc;
+        int ybase = iy[j0 + (dir > 0 ? 0 : nrows - 1)];
+        float2 acc[W];
+#pragma unroll
+        for (int b = 0; b < W; ++b) acc[b] = make_float2(0.f, 0.f);
+        for (int t = 0; t < nrows; ++t) {
+            const int r = dir > 0 ? t : nrows - 1 - t;
+            const int j = j0 + r;
+            const int iyr = iy[j];
+            while (ybase < iyr) {
+                float2 *dst = col + (long long)wrap_any(ybase, nfy) * nxc;
+                *dst = cadd(*dst, acc[0]);
+#pragma unroll
+                for (int b = 0; b < W - 1; ++b) acc[b] = acc[b + 1];
+                acc[W - 1] = make_float2(0.f, 0.f);
+                ++ybase;
+            }
+            const unsigned *cj = count + (long long)j * rg.ntiles;
+            const float2 *hr = halo + (long long)r * rg.ntiles * rg.NH * 3;
+            float2 h = make_float2(0.f, 0.f);
+            if (cj[t0]) h = slab[((long long)r * nfz + z) * nxc + xc];
+            if (south && cj[tS]) h = cadd(h, hr[((long long)tS * rg.NH + hS) * 3 + m]);
+            if (east && cj[tE]) h = cadd(h, hr[((long long)tE * rg.NH + hE) * 3 + m]);
+            if (south && east && cj[tC]) h = cadd(h, hr[((long long)tC * rg.NH + hC) * 3 + m]);
+            const float *w = wy + (long long)j * W;
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                acc[b].x += w[b] * h.x;
+                acc[b].y += w[b] * h.y;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            float2 *dst = col + (long long)wrap_any(ybase + b, nfy) * nxc;
+            *dst = cadd(*dst, acc[b]);
+        }
+    }
+}
+
+// Box of a 2-D tile: cells [o, o + HB) per axis.
+struct Box2 {
+    int r, tile, oz, ox, ncell;
+};
+
+__device__ __forceinline__ Box2 box2(unsigned jt, int j0, const RowGeom &rg) {
+    Box2 b;
+    const int j = (int)(jt / rg.ntiles);
+    b.tile = (int)(jt - (unsigned)j * rg.ntiles);
+    const int tz = b.tile / rg.ntx;
+    b.r = j - j0;
+    b.oz = tz * rg.Tz;
+    b.ox = (b.tile - tz * rg.ntx) * rg.Tx;
+    b.ncell = rg.HBz * rg.HBx;
+    return b;
+}
+
+// type 2, step 2: the chunk's 2-D box of its row's slab is staged in LDS
+// ([ncell][3]); each point gathers W^2 x 3 values and writes sum_i d_i c_i and
+// its mirror.
+template <int W>
+__global__ __launch_bounds__(256) void k_rows_interp(const float2 *__restrict__ slab, float2 *__restrict__ c,
+                                                     const Geo *__restrict__ geo, long long P, int R, int C,
+                                                     int nfz, int nfx, float beta, PointFrame fr, RowGeom rg,
+                                                     int j0, const unsigned *__restrict__ ids,
+                                                     const uint4 *__restrict__ chunks) {
+    extern __shared__ float lds[];
+    const uint4 ch = chunks[blockIdx.x];
+    const Box2 bx = box2(ch.x, j0, rg);
+    const float2 *rs = slab + (long long)bx.r * nfz * nfx * 3;
+    float2 *box = reinterpret_cast<float2 *>(lds);
+    for (int i = threadIdx.x; i < 3 * bx.ncell; i += blockDim.x) {
+        const int cell = i / 3, q = i - cell * 3;
+        const int cz = (int)rg.fbx.div((unsigned)cell), cxx = cell - cz * rg.HBx;
+        const int gz = wrap_up(bx.oz + cz, nfz), gx = wrap_up(bx.ox + cxx, nfx);
+        box[i] = rs[((long long)gz * nfx + gx) * 3 + q];
+    }
+    __syncthreads();
+    const long long S = (long long)R * C, H = (S + 1) / 2;
+    for (unsigned i = threadIdx.x; i < ch.z; i += blockDim.x) {
+        const long long gid = ids[ch.y + i];
+        const long long p = gid / H;
+        const long long idx = gid - p * H;
+        const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
+        const Geo g = geo[p];
+        double tx, ty, tz;
+        point_coords(g, fr, j, k, tx, ty, tz);
+        const int ix = first_cell(tx, nfx, fr.rx, W), iz = first_cell(tz, nfz, fr.rz, W);
+        float wz[W], wx[W];
+        es_weights<W>(tz, iz, beta, wz);
+        es_weights<W>(tx, ix, beta, wx);
+        const int lz = wrap0(iz, nfz) - bx.oz, lx = wrap0(ix, nfx) - bx.ox;
+        float2 acc[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = make_float2(0.f, 0.f);
+        if (lz >= 0 && lz < rg.Tz && lx >= 0 && lx < rg.Tx) {
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                const float2 *row = box + (size_t)((lz + a) * rg.HBx + lx) * 3;
+                float2 racc[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) racc[q] = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const float2 v = row[e * 3 + q];
+                        racc[q].x += v.x * wx[e];
+                        racc[q].y += v.y * wx[e];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    acc[q].x += racc[q].x * wz[a];
+                    acc[q].y += racc[q].y * wz[a];
+                }
+            }
+        } else {
+            // a point outside its tile (not expected): gather from the slab
+            for (int a = 0; a < W; ++a) {
+                const int zz = wrapi(iz + a, nfz);
+                for (int e = 0; e < W; ++e) {
+                    const float2 *v = rs + ((long long)zz * nfx + wrapi(ix + e, nfx)) * 3;
+                    const float wgt = wz[a] * wx[e];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        acc[q].x += v[q].x * wgt;
+                        acc[q].y += v[q].y * wgt;
+                    }
+                }
+            }
+        }
+        store_half_point(c, p, idx, S, g, acc);
+    }
+}
+
+// type 1, step 1: one wave per (row, tile) accumulates its 2-D box ([3][ncell]
+// in LDS).  Lane l < W^2 owns cell (l / W, l % W) of a point's stamp; points
+// come sorted by first kernel cell, so a run of points sharing one is summed
+// in registers and added to the box once.  The box's interior goes to the
+// slab and the rest to the tile's halo record, both with plain stores.
+constexpr int kStageRows = 64;
+
+template <int W>
+struct RowStage {
+    static constexpr int VQ = 2;             // [0] box cell, [1] unused, values, wz[W], wx[W]
+    static constexpr int WZ = VQ + 6;
+    static constexpr int WX = WZ + W;
+    static constexpr int SW = (WX + W + 3) / 4 * 4;
+};
+
+template <int W>
+__global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, float2 *__restrict__ halo,
+                                                    const float2 *__restrict__ c, const Geo *__restrict__ geo,
+                                                    long long P, int R, int C, int nfz, int nfx, float beta,
+                                                    PointFrame fr, RowGeom rg, int j0,
+                                                    const unsigned *__restrict__ ids,
+                                                    const uint4 *__restrict__ chunks, int paired) {
+    using SL = RowStage<W>;
+    constexpr int SW = SL::SW;
+    constexpr int NST = W * W;
+    constexpr int NCH = (NST + 63) / 64;
+    extern __shared__ float lds[];
+    const uint4 ch = chunks[blockIdx.x];
+    const Box2 bx = box2(ch.x, j0, rg);
+    float *stg = lds;                                                        // [kStageRows][SW]
+    float2 *acc = reinterpret_cast<float2 *>(lds + (size_t)kStageRows * SW);  // [3][ncell]
+    for (int i = threadIdx.x; i < 3 * bx.ncell; i += blockDim.x) acc[i] = make_float2(0.f, 0.f);
+    const int lane = threadIdx.x;
+    const long long S = (long long)R * C, H = (S + 1) / 2;
+    const int HB = rg.HBx;
+
+    int loff[NCH], wzo[NCH], wxo[NCH];
+    bool act[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+        const int l = lane + 64 * q;
+        act[q] = l < NST;
+        const int cc = act[q] ? l : 0;
+        const int a = cc / W, e = cc - (cc / W) * W;
+        loff[q] = a * HB + e;
+        wzo[q] = SL::WZ + a;
+        wxo[q] = SL::WX + e;
+    }
+    float2 run[NCH][3];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) run[q][m] = make_float2(0.f, 0.f);
+    int run_cell = -1;
+    auto add_run = [&]() {
+        if (run_cell < 0) return;
+        float2 t[NCH][3];
+#pragma unroll
+        for (int q = 0; q < NCH; ++q)
+            if (act[q]) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) t[q][m] = acc[m * bx.ncell + run_cell + loff[q]];
+            }
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            if (act[q]) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) acc[m * bx.ncell + run_cell + loff[q]] = cadd(t[q][m], run[q][m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 3; ++m) run[q][m] = make_float2(0.f, 0.f);
+        }
+    };
+
+    for (unsigned base = 0; base < ch.z; base += kStageRows) {
+        const int nb = (int)min((unsigned)kStageRows, ch.z - base);
+        __syncthreads();   // the box is zeroed / the previous batch is consumed
+        if (lane < nb) {
+            const long long gid = ids[ch.y + base + lane];
+            const long long p = gid / H;
+            const long long idx = gid - p * H;
+            const int j = (int)(idx / C), k = (int)(idx - (long long)j * C);
+            const Geo g = geo[p];
+            float2 vc[3];
+            half_point_values(c, p, idx, S, g, paired, vc);
+            double tx, ty, tz;
+            point_coords(g, fr, j, k, tx, ty, tz);
+            const int ix = first_cell(tx, nfx, fr.rx, W), iz = first_cell(tz, nfz, fr.rz, W);
+            float wz[W], wx[W];
+            es_weights<W>(tz, iz, beta, wz);
+            es_weights<W>(tx, ix, beta, wx);
+            int lz = wrap0(iz, nfz) - bx.oz, lx = wrap0(ix, nfx) - bx.ox;
+            // the binning placed the point in this tile with the same (uncontracted FP64)
+            // arithmetic, so it lies inside; the clamp only guarantees that no LDS access
+            // can leave the box
+            lz = min(max(lz, 0), rg.Tz - 1);
+            lx = min(max(lx, 0), rg.Tx - 1);
+            float *sp = stg + lane * SW;
+            reinterpret_cast<int *>(sp)[0] = lz * HB + lx;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                sp[SL::VQ + 2 * m] = vc[m].x;
+                sp[SL::VQ + 2 * m + 1] = vc[m].y;
+            }
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                sp[SL::WZ + a] = wz[a];
+                sp[SL::WX + a] = wx[a];
+            }
+        }
+        __syncthreads();
+
+        struct Rec {
+            int hdr;
+            float2 v[3];
+            float wz[NCH], wx[NCH];
+        };
+        auto fetch = [&](int kk, Rec &r) {
+            const float *sp = stg + kk * SW;
+            r.hdr = reinterpret_cast<const int *>(sp)[0];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) r.v[m] = *reinterpret_cast<const float2 *>(sp + SL::VQ + 2 * m);
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) {
+                r.wz[q] = sp[wzo[q]];
+                r.wx[q] = sp[wxo[q]];
+            }
+        };
+        auto accumulate = [&](const Rec &r) {
+            const int cb = __builtin_amdgcn_readfirstlane(r.hdr);
+            if (cb != run_cell) {
+                add_run();
+                run_cell = cb;
+            }
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) {
+                const float w = r.wz[q] * r.wx[q];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    run[q][m].x += r.v[m].x * w;
+                    run[q][m].y += r.v[m].y * w;
+                }
+            }
+        };
+        Rec ra, rb;
+        fetch(0, ra);
+        for (int kk = 0; kk < nb; kk += 2) {
+            if (kk + 1 < nb) fetch(kk + 1, rb);
+            accumulate(ra);
+            if (kk + 1 < nb) {
+                if (kk + 2 < nb) fetch(kk + 2, ra);
+                accumulate(rb);
+            }
+        }
+    }
+    add_run();
+    __syncthreads();
+    float2 *rs = slab + (long long)bx.r * nfz * nfx * 3;
+    float2 *hrec = halo + ((long long)bx.r * rg.ntiles + bx.tile) * rg.NH * 3;
+    for (int cell = threadIdx.x; cell < bx.ncell; cell += blockDim.x) {
+        const int cz = (int)rg.fbx.div((unsigned)cell), cxx = cell - cz * HB;
+        float2 *dst;
+        if (cz < rg.Tz && cxx < rg.Tx) {
+            dst = rs + ((long long)(bx.oz + cz) * nfx + (bx.ox + cxx)) * 3;
+        } else {
+            int h;
+            if (cz >= rg.Tz && cxx < rg.Tx) h = (cz - rg.Tz) * rg.Tx + cxx;
+            else if (cz < rg.Tz) h = (W - 1) * rg.Tx + cz * (W - 1) + (cxx - rg.Tx);
+            else h = (W - 1) * rg.Tx + rg.Tz * (W - 1) + (cz - rg.Tz) * (W - 1) + (cxx - rg.Tx);
+            dst = hrec + (long long)h * 3;
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dst[m] = acc[m * bx.ncell + cell];
     }
 }
 
@@ -775,6 +1198,233 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
     return ref;
 }
 
+// ---- row-factorised path (host)
+constexpr int kTileRowsTarget = 16;     // preferred 2-D tile edge of the row path (measured: 16 < 20 < 24)
+
+// Slab memory of one row group: LAMINO_NUFFT_SLAB_BYTES, default 1 GiB.
+size_t slab_budget() {
+    const char *e = std::getenv("LAMINO_NUFFT_SLAB_BYTES");
+    if (e && *e) {
+        const long long v = std::atoll(e);
+        if (v > 0) return (size_t)v;
+    }
+    return (size_t)1 << 30;
+}
+
+long long half_rows(const Grid &g) {
+    const long long H = (g.R * g.C + 1) / 2;
+    return (H - 1) / g.C + 1;
+}
+
+// a tile edge that divides n, at least w (the halo reaches one tile only), closest to the target
+int row_tile(int n, int w) {
+    int best = 0;
+    for (int d = w; d <= std::min(n, 48); ++d)
+        if (n % d == 0 && (best == 0 || std::abs(d - kTileRowsTarget) < std::abs(best - kTileRowsTarget))) best = d;
+    return best;
+}
+
+RowGeom row_geom(const NufftPlan &pl) {
+    RowGeom rg;
+    const int W = pl.w;
+    rg.Tz = row_tile(pl.nf[0], W);
+    rg.Tx = row_tile(pl.nf[2], W);
+    if (!rg.Tz || !rg.Tx) return rg;
+    rg.ntz = pl.nf[0] / rg.Tz;
+    rg.ntx = pl.nf[2] / rg.Tx;
+    rg.ntiles = (unsigned)(rg.ntz * rg.ntx);
+    rg.HBz = rg.Tz + W - 1;
+    rg.HBx = rg.Tx + W - 1;
+    rg.NH = (W - 1) * rg.Tx + rg.Tz * (W - 1) + (W - 1) * (W - 1);
+    rg.fbx = FastDiv((unsigned)rg.HBx);
+    return rg;
+}
+
+// The three-component projectors take the row path when every projection
+// shares the tilt (LAMINO_NUFFT_ROWS=0 turns it off), tile edges dividing the
+// fine grid exist and the sort keys fit 32 bits.
+bool use_rows(const Grid &g, const NufftPlan &pl) {
+    if (!g.shared_tilt) return false;
+    const char *e = std::getenv("LAMINO_NUFFT_ROWS");
+    if (e && e[0] == '0') return false;
+    const RowGeom rg = row_geom(pl);
+    if (!rg.Tz) return false;
+    return half_rows(g) * (long long)rg.ntiles * rg.Tz * rg.Tx <= (1ll << 32);
+}
+
+size_t rows_spread_lds(const RowGeom &rg, int w) {
+    const size_t sw = (size_t)(2 + 6 + 2 * w + 3) / 4 * 4;   // RowStage<w>::SW
+    return (size_t)kStageRows * sw * 4 + 3 * (size_t)rg.HBz * rg.HBx * 8;
+}
+size_t rows_interp_lds(const RowGeom &rg) { return 3 * (size_t)rg.HBz * rg.HBx * 8; }
+
+RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
+    std::vector<long long> key = {pl.nf[0], pl.nf[1], pl.nf[2], pl.w};
+    auto it = g.rowplans.find(key);
+    if (it != g.rowplans.end()) return *it->second;
+    TimedScope ts(cx, "bin_points");
+    const int W = pl.w;
+    const long long S = g.R * g.C, H = (S + 1) / 2, total = g.P * H;
+    LAM_REQUIRE(total < (1ll << 31), LAM_ERR_ARG, "too many points in one polar grid for 32-bit bins");
+    auto rp = std::make_unique<RowPlan>();
+    rp->rg = row_geom(pl);
+    const RowGeom &rg = rp->rg;
+    LAM_REQUIRE(rg.Tz > 0, LAM_ERR_STATE, "no row tiling for this fine grid");
+    rp->Jn = (int)half_rows(g);
+    const long long nkeys = (long long)rp->Jn * rg.ntiles * rg.Tz * rg.Tx;
+    LAM_REQUIRE(nkeys <= (1ll << 32), LAM_ERR_ARG, "fine grid too large for 32-bit row bin keys");
+
+    // Row weights from the same coordinate expression as the kernels (ty = (qY vy) sy),
+    // in double; iy is the unwrapped first cell, linear in j and so monotonic.
+    const PointFrame fr = frame(g, pl);
+    const double vy = g.geo_host[0].vy;
+    std::vector<float> wyh((size_t)rp->Jn * W);
+    std::vector<int> iyh(rp->Jn);
+    for (int j = 0; j < rp->Jn; ++j) {
+        const double qY = ((double)j - fr.j0) * fr.dY;
+        const double ty = (qY * vy) * fr.sy;
+        const int iy0 = (int)std::ceil(ty - 0.5 * W);
+        iyh[j] = iy0;
+        for (int m = 0; m < W; ++m) {
+            const double z = ((double)(iy0 + m) - ty) * (2.0 / W);
+            const double r = 1.0 - z * z;
+            wyh[(size_t)j * W + m] = r > 0 ? (float)std::exp((double)pl.beta * (std::sqrt(r) - 1.0)) : 0.f;
+        }
+    }
+    rp->dir = iyh[rp->Jn - 1] >= iyh[0] ? 1 : -1;
+    for (int j = 0; j + 1 < rp->Jn; ++j)
+        LAM_REQUIRE(rp->dir * (iyh[j + 1] - iyh[j]) >= 0, LAM_ERR_STATE, "row first cells are not monotonic");
+    rp->wy.alloc(sizeof(float) * wyh.size());
+    upload_now(cx, rp->wy.ptr, wyh.data(), sizeof(float) * wyh.size());
+    rp->iy.alloc(sizeof(int) * iyh.size());
+    upload_now(cx, rp->iy.ptr, iyh.data(), sizeof(int) * iyh.size());
+
+    int key_bits = 1;
+    while (key_bits < 32 && (1ll << key_bits) < nkeys) ++key_bits;
+    const long long ncount = (long long)rp->Jn * rg.ntiles;
+    DevBuf key0(sizeof(unsigned) * total), key1(sizeof(unsigned) * total);
+    DevBuf val0(sizeof(unsigned) * total), val1(sizeof(unsigned) * total);
+    rp->count.alloc(sizeof(unsigned) * ncount);
+    LAM_HIP(hipMemsetAsync(rp->count.ptr, 0, rp->count.bytes, cx.stream));
+    const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
+    LAM_WIDTH_SWITCH(W, hipLaunchKernelGGL((k_bin_keys_rows<W>), dim3(grid), dim3(256), 0, cx.stream,
+                                           g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[2], fr, rg,
+                                           key0.as<unsigned>(), val0.as<unsigned>(), rp->count.as<unsigned>()));
+    check_launch();
+    hipcub::DoubleBuffer<unsigned> dk(key0.as<unsigned>(), key1.as<unsigned>());
+    hipcub::DoubleBuffer<unsigned> dv(val0.as<unsigned>(), val1.as<unsigned>());
+    size_t temp_bytes = 0;
+    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
+    DevBuf temp(std::max<size_t>(temp_bytes, 16));
+    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
+    rp->ids = std::move(dv.Current() == val0.as<unsigned>() ? val0 : val1);
+    std::vector<unsigned> hc(ncount);
+    LAM_HIP(hipMemcpyAsync(hc.data(), rp->count.ptr, sizeof(unsigned) * ncount, hipMemcpyDeviceToHost, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+    // one chunk per non-empty (row, tile): its workgroup alone writes that tile's
+    // interior and halo record
+    std::vector<uint4> chunks;
+    rp->row_chunk.assign((size_t)rp->Jn + 1, 0);
+    unsigned long long off = 0;
+    for (long long jt = 0; jt < ncount; ++jt) {
+        if (jt % rg.ntiles == 0) rp->row_chunk[jt / rg.ntiles] = (long long)chunks.size();
+        if (hc[jt]) chunks.push_back(make_uint4((unsigned)jt, (unsigned)off, hc[jt], 0u));
+        off += hc[jt];
+    }
+    rp->nchunks = (long long)chunks.size();
+    rp->row_chunk[rp->Jn] = rp->nchunks;
+    if (!chunks.empty()) {
+        rp->chunks.alloc(sizeof(uint4) * chunks.size());
+        LAM_HIP(hipMemcpyAsync(rp->chunks.ptr, chunks.data(), sizeof(uint4) * chunks.size(), hipMemcpyHostToDevice,
+                               cx.stream));
+        LAM_HIP(hipStreamSynchronize(cx.stream));
+    }
+    RowPlan &ref = *rp;
+    g.rowplans.emplace(std::move(key), std::move(rp));
+    return ref;
+}
+
+size_t row_slab_bytes(const NufftPlan &pl) { return (size_t)pl.nf[0] * pl.nf[2] * 3 * sizeof(float2); }
+size_t row_halo_bytes(const RowGeom &rg) { return (size_t)rg.ntiles * rg.NH * 3 * sizeof(float2); }
+
+// rows per group: as many slabs (and halo records) as the budget holds
+int rows_per_group(const RowPlan &rp, const NufftPlan &pl) {
+    const size_t per_row = row_slab_bytes(pl) + row_halo_bytes(rp.rg);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)rp.Jn, slab_budget() / per_row));
+}
+
+}  // namespace
+
+// Upper bound of what the row path allocates for one grid (see nufft.h).
+RowBytes row_path_bytes(const long long dims[3], double eps, long long P, long long R, long long C) {
+    NufftPlan pl;   // shapes only
+    pl.w = width_for_eps(eps);
+    for (int a = 0; a < 3; ++a) pl.nf[a] = (int)nufft_fine_len(dims[a], eps);
+    RowBytes b;
+    const RowGeom rg = row_geom(pl);
+    if (!rg.Tz) return b;
+    const long long H = (R * C + 1) / 2, Jn = (H - 1) / C + 1, half = P * H;
+    const size_t per_row = row_slab_bytes(pl) + row_halo_bytes(rg);
+    const size_t groups_rows = std::max<size_t>(1, std::min<size_t>((size_t)Jn, slab_budget() / per_row));
+    b.slab = groups_rows * per_row;
+    const size_t cells = (size_t)Jn * rg.ntiles;
+    b.kept = 4 * (size_t)half + 20 * cells + 8 * (size_t)Jn * pl.w;   // ids, counts + chunks, row weights
+    b.setup = 16 * (size_t)half + ((size_t)16 << 20);                // sort keys / spare ids, sort scratch
+    return b;
+}
+
+namespace {
+
+template <int W>
+void interp_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, RowPlan &rp) {
+    const int Jg = rows_per_group(rp, pl);
+    pl.slab.ensure(row_slab_bytes(pl) * Jg);
+    const PointFrame fr = frame(g, pl);
+    const size_t lds = rows_interp_lds(rp.rg);
+    allow_lds(&k_rows_interp<W>, lds);
+    const unsigned cgrid = grid_for((size_t)pl.nf[0] * pl.nf[2] * 3, 256, 1u << 20);
+    for (int j0 = 0; j0 < rp.Jn; j0 += Jg) {
+        const int nrows = std::min(Jg, rp.Jn - j0);
+        hipLaunchKernelGGL((k_rows_contract<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
+                           pl.slab.as<float2>(), rp.wy.as<float>(), rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2],
+                           j0, nrows, rp.dir);
+        check_launch();
+        const long long c0 = rp.row_chunk[j0], c1 = rp.row_chunk[j0 + nrows];
+        if (c1 > c0) {
+            hipLaunchKernelGGL((k_rows_interp<W>), dim3((unsigned)(c1 - c0)), dim3(256), lds, cx.stream,
+                               pl.slab.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0],
+                               pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(), rp.chunks.as<uint4>() + c0);
+            check_launch();
+        }
+    }
+}
+
+template <int W>
+void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp, bool paired) {
+    const int Jg = rows_per_group(rp, pl);
+    pl.slab.ensure(row_slab_bytes(pl) * Jg);
+    pl.halo.ensure(row_halo_bytes(rp.rg) * Jg);
+    const PointFrame fr = frame(g, pl);
+    const size_t lds = rows_spread_lds(rp.rg, W);
+    allow_lds(&k_rows_spread<W>, lds);
+    const unsigned cgrid = grid_for((size_t)pl.nf[0] * pl.nf[2] * 3, 256, 1u << 20);
+    for (int j0 = 0; j0 < rp.Jn; j0 += Jg) {
+        const int nrows = std::min(Jg, rp.Jn - j0);
+        const long long c0 = rp.row_chunk[j0], c1 = rp.row_chunk[j0 + nrows];
+        if (c1 > c0) {
+            hipLaunchKernelGGL((k_rows_spread<W>), dim3((unsigned)(c1 - c0)), dim3(64), lds, cx.stream,
+                               pl.slab.as<float2>(), pl.halo.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R,
+                               (int)g.C, pl.nf[0], pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(),
+                               rp.chunks.as<uint4>() + c0, paired ? 1 : 0);
+            check_launch();
+        }
+        hipLaunchKernelGGL((k_rows_expand<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
+                           pl.slab.as<float2>(), pl.halo.as<float2>(), rp.count.as<unsigned>(), rp.wy.as<float>(),
+                           rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2], rp.rg, j0, nrows, rp.dir);
+        check_launch();
+    }
+}
+
 template <int W, bool HALF>
 void spread_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, const BinParams &bp, SortedPoints &sp, bool paired) {
     constexpr int NC = HALF ? 3 : 1;
@@ -798,6 +1448,13 @@ void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, S
 // paired: c holds detector-pair spectra (det_fft2_real_pairs), three-component form only
 void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, bool paired = false) {
     const int nc = half ? 3 : 1;
+    if (half && use_rows(g, pl)) {
+        RowPlan &rp = get_rowplan(cx, g, pl);
+        LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * 3, cx.stream));
+        TimedScope ts(cx, "spread");
+        LAM_WIDTH_SWITCH(pl.w, (spread_rows_w<W>(cx, g, pl, c, rp, paired)));
+        return;
+    }
     const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
     LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc, cx.stream));
@@ -813,6 +1470,12 @@ void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, 
 
 void launch_interp(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, bool half) {
     const int nc = half ? 3 : 1;
+    if (half && use_rows(g, pl)) {
+        RowPlan &rp = get_rowplan(cx, g, pl);
+        TimedScope ts(cx, "interp");
+        LAM_WIDTH_SWITCH(pl.w, (interp_rows_w<W>(cx, g, pl, c, rp)));
+        return;
+    }
     const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
     if (sp.nchunks == 0) return;
@@ -972,6 +1635,7 @@ std::unique_ptr<Grid> make_grid(Ctx &cx, const double *alpha, const double *gamm
     g->P = P; g->R = R; g->C = C;
     g->alpha.assign(alpha, alpha + P);
     g->gamma.assign(gamma, gamma + P);
+    g->shared_tilt = std::all_of(alpha, alpha + P, [&](double a) { return a == alpha[0]; });
     g->geo_host.resize(P);
     for (long long p = 0; p < P; ++p) {
         const double a = alpha[p], gm = gamma[p];
@@ -1068,6 +1732,54 @@ bool bins_tuned(Ctx &cx, Grid &g, const long long dims[3], double eps) {
 
 std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms) {
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
+    if (use_rows(g, pl)) {
+        // The projectors of this grid take the row path: its tiling follows from the
+        // fine grid, so there is nothing to choose.  Report it as (Tz, W, Tx) with the
+        // time of one spread + interpolation, and mark the grid as tuned.
+        ensure_fine(pl, 3);
+        const size_t M = (size_t)(g.P * g.R * g.C);
+        cx.work[0].ensure(sizeof(float2) * M);
+        float2 *cbuf = cx.work[0].as<float2>();
+        LAM_HIP(hipMemsetAsync(cbuf, 0, sizeof(float2) * M, cx.stream));
+        const RowGeom rg = row_geom(pl);
+        std::array<int, 3> T = {rg.Tz, pl.w, rg.Tx};
+        hipEvent_t e0, e1;
+        LAM_HIP(hipEventCreate(&e0));
+        LAM_HIP(hipEventCreate(&e1));
+        const bool timing = cx.timer.on;
+        cx.timer.on = false;
+        float t = 0.f;
+        try {
+            launch_spread(cx, g, pl, cbuf, true);   // builds the row bins
+            for (int rep = 0; rep < 2; ++rep) {
+                LAM_HIP(hipEventRecord(e0, cx.stream));
+                launch_spread(cx, g, pl, cbuf, true);
+                launch_interp(cx, g, pl, cbuf, true);
+                LAM_HIP(hipEventRecord(e1, cx.stream));
+                LAM_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                LAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                t = rep == 0 ? ms : std::min(t, ms);
+            }
+        } catch (...) {
+            cx.timer.on = timing;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        cx.timer.on = timing;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const BinParams bp = bin_params(cx, g, pl, 3);
+        const std::array<int, 3> T3 = {bp.T[0], bp.T[1], bp.T[2]};
+        cx.bin_tune[tune_key(g, pl, 3)] = T3;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[tune_key(g, pl, 3)] = T3;
+        }
+        if (best_ms) *best_ms = t;
+        return T;
+    }
     ensure_fine(pl, 3);
     const size_t M = (size_t)(g.P * g.R * g.C);
     cx.work[0].ensure(sizeof(float2) * M);
