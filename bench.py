@@ -229,7 +229,8 @@ def main():
         return
 
     kernels = {}
-    for name in ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points"):
+    for name in ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points",
+                 "spread_rows2d", "y_expand", "y_contract", "interp_rows2d"):
         tot, cnt = ctypes.c_double(), ctypes.c_int64()
         _lib.check(lib.lam_timing_query(ctx.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt)))
         if cnt.value:
@@ -262,19 +263,30 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic_from_profiles(dom), "kernel": dom,
                 "bytes_per_launch": bytes_alg, "avg_launch_ms": round(kernels[dom]["avg_ms"], 4)}
-    # The roof that binds the NUFFT kernels is LDS, not HBM: per half-plane point the
-    # spreader read-modify-writes w^3 cells x 3 complex components (2 x 24 B each) in
-    # its LDS box and the interpolator reads them once.  Peak = 256 CUs x 128 B/clk
-    # x 2.4 GHz (an estimate from the CU's LDS width, not a published figure).
+    # The roofs that bind the NUFFT kernels.  These datasets share their tilt, so the
+    # projectors run row-factorised: per half-plane point the 2-D spreader
+    # read-modify-writes W^2 cells x 3 complex components (2 x 24 B each) in its LDS
+    # box and the 2-D interpolator reads them once (LDS roof; peak = 256 CUs x 128 B/clk
+    # x 2.4 GHz, an estimate from the CU's LDS width, not a published figure); the y
+    # passes stream the row slabs and the fine grid (HBM roof): y_contract reads the
+    # fine grid once and writes one (z, x) slab per row, y_expand reads the slabs
+    # (plus halo strips, not counted) and read-modify-writes the fine grid.
     w = 6
-    half_points = G * P * ((det * det + 1) // 2) // G   # per launch: one dataset
+    half_points = P * ((det * det + 1) // 2)           # per launch: one dataset
+    rows = ((det * det + 1) // 2 - 1) // det + 1        # detector rows of the half set
+    slab = nf[0] * nf[2] * 24
     lds_peak_tbs = 256 * 128 * 2.4e9 / 1e12
     binding = {}
-    for k, per_pt in (("spread", 2 * 24 * w ** 3), ("interp", 24 * w ** 3)):
+    for k, per_pt in (("spread_rows2d", 2 * 24 * w ** 2), ("interp_rows2d", 24 * w ** 2)):
         if k in kernels:
             tbs = half_points * per_pt / (kernels[k]["avg_ms"] / 1e3) / 1e12
             binding[k] = {"roof": "lds", "achieved_tbs": round(tbs, 2), "peak_tbs": round(lds_peak_tbs, 1),
                           "frac": round(tbs / lds_peak_tbs, 3), "bytes_per_launch": half_points * per_pt}
+    for k, nbytes in (("y_contract", Nfine * 24 + rows * slab), ("y_expand", rows * slab + 2 * Nfine * 24)):
+        if k in kernels:
+            gbs = nbytes / (kernels[k]["avg_ms"] / 1e3) / 1e9
+            binding[k] = {"roof": "hbm", "achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK_GBS,
+                          "frac": round(gbs / HBM_PEAK_GBS, 3), "bytes_per_launch": nbytes}
     roofline["binding"] = binding
 
     # CG iterations/s: Toeplitz normal operator + ramp preconditioner at the same shape

@@ -1385,12 +1385,16 @@ void interp_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, RowPlan &rp) {
     const unsigned cgrid = grid_for((size_t)pl.nf[0] * pl.nf[2] * 3, 256, 1u << 20);
     for (int j0 = 0; j0 < rp.Jn; j0 += Jg) {
         const int nrows = std::min(Jg, rp.Jn - j0);
-        hipLaunchKernelGGL((k_rows_contract<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
-                           pl.slab.as<float2>(), rp.wy.as<float>(), rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2],
-                           j0, nrows, rp.dir);
-        check_launch();
+        {
+            TimedScope ts(cx, "y_contract");
+            hipLaunchKernelGGL((k_rows_contract<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
+                               pl.slab.as<float2>(), rp.wy.as<float>(), rp.iy.as<int>(), pl.nf[0], pl.nf[1],
+                               pl.nf[2], j0, nrows, rp.dir);
+            check_launch();
+        }
         const long long c0 = rp.row_chunk[j0], c1 = rp.row_chunk[j0 + nrows];
         if (c1 > c0) {
+            TimedScope ts(cx, "interp_rows2d");
             hipLaunchKernelGGL((k_rows_interp<W>), dim3((unsigned)(c1 - c0)), dim3(256), lds, cx.stream,
                                pl.slab.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0],
                                pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(), rp.chunks.as<uint4>() + c0);
@@ -1412,12 +1416,14 @@ void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp
         const int nrows = std::min(Jg, rp.Jn - j0);
         const long long c0 = rp.row_chunk[j0], c1 = rp.row_chunk[j0 + nrows];
         if (c1 > c0) {
+            TimedScope ts(cx, "spread_rows2d");
             hipLaunchKernelGGL((k_rows_spread<W>), dim3((unsigned)(c1 - c0)), dim3(64), lds, cx.stream,
                                pl.slab.as<float2>(), pl.halo.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R,
                                (int)g.C, pl.nf[0], pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(),
                                rp.chunks.as<uint4>() + c0, paired ? 1 : 0);
             check_launch();
         }
+        TimedScope ts(cx, "y_expand");
         hipLaunchKernelGGL((k_rows_expand<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
                            pl.slab.as<float2>(), pl.halo.as<float2>(), rp.count.as<unsigned>(), rp.wy.as<float>(),
                            rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2], rp.rg, j0, nrows, rp.dir);
