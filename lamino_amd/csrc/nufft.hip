@@ -705,13 +705,20 @@ __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, 
                 acc[W - 1] = make_float2(0.f, 0.f);
                 ++ybase;
             }
+            // Every address below lies inside the buffers whether or not its tile holds
+            // points, so all loads are issued at once and the empty tiles' values (never
+            // written) are masked out afterwards instead of branching on the counts.
             const unsigned *cj = count + (long long)j * rg.ntiles;
             const float2 *hr = halo + (long long)r * rg.ntiles * rg.NH * 3;
-            float2 h = make_float2(0.f, 0.f);
-            if (cj[t0]) h = slab[((long long)r * nfz + z) * nxc + xc];
-            if (south && cj[tS]) h = cadd(h, hr[((long long)tS * rg.NH + hS) * 3 + m]);
-            if (east && cj[tE]) h = cadd(h, hr[((long long)tE * rg.NH + hE) * 3 + m]);
-            if (south && east && cj[tC]) h = cadd(h, hr[((long long)tC * rg.NH + hC) * 3 + m]);
+            const unsigned n0 = cj[t0], nS = cj[tS], nE = cj[tE], nC = cj[tC];
+            const float2 v0 = slab[((long long)r * nfz + z) * nxc + xc];
+            const float2 vS = hr[((long long)tS * rg.NH + (south ? hS : 0)) * 3 + m];
+            const float2 vE = hr[((long long)tE * rg.NH + (east ? hE : 0)) * 3 + m];
+            const float2 vC = hr[((long long)tC * rg.NH + (south && east ? hC : 0)) * 3 + m];
+            float2 h = n0 ? v0 : make_float2(0.f, 0.f);
+            if (south && nS) h = cadd(h, vS);
+            if (east && nE) h = cadd(h, vE);
+            if (south && east && nC) h = cadd(h, vC);
             const float *w = wy + (long long)j * W;
 #pragma unroll
             for (int b = 0; b < W; ++b) {
