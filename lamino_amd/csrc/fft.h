@@ -40,6 +40,9 @@ struct AxisPass {
     int c1 = 1, c2 = 1, ci = 1;
     int nv = -1, ns = -1;      // -1: full length; valid loads m < nv, stored outputs m < ns
     int nv2 = 0, ns2 = 0;      // additionally valid / stored: the last nv2 / ns2 positions (wrapped ranges)
+    // windows: when *_len > 0, only positions m with (m - *_lo) mod n < *_len are loaded (the
+    // rest read as zero) / stored
+    int ld_lo = 0, ld_len = 0, st_lo = 0, st_len = 0;
     int rot_in = 0, rot_out = 0;
     // pair modes: lines with o1 >= c1_in2 have no second input (in2 reads as zero),
     // lines with o1 >= c1_out2 store no second output

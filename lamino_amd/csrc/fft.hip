@@ -149,6 +149,14 @@ __device__ void stages(float2 *buf, const KParams &P, int sign) {
 
 __device__ __forceinline__ int remap(int o, int split, int gap) { return o < split ? o : o + gap; }
 
+// position m in the cyclic window [lo, lo + len) of a length-n line (len 0: no window)
+__device__ __forceinline__ bool in_window(int m, int lo, int len, int n) {
+    if (len <= 0) return true;
+    int d = m - lo;
+    if (d < 0) d += n;
+    return d < len;
+}
+
 // Global coordinates of tile line l; false when the line lies past the set.
 __device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, int ibase, long long &o1, int &o2,
                                           int &i) {
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         if (mm >= N) mm -= N;
         float2 v = make_float2(0.f, 0.f);
         const long long lb = base_in[l];
-        if (lb >= 0 && (mm < nv || mm >= N - a.nv2)) {
+        if (lb >= 0 && (mm < nv || mm >= N - a.nv2) && in_window(mm, a.ld_lo, a.ld_len, N)) {
             const long long off = lb + (long long)mm * P.isn;
             if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
             else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         int mm = n + a.rot_out;
         if (mm >= N) mm -= N;
         const long long lb = base_out[l];
-        if (lb >= 0 && (mm < ns || mm >= N - a.ns2)) {
+        if (lb >= 0 && (mm < ns || mm >= N - a.ns2) && in_window(mm, a.st_lo, a.st_len, N)) {
             float2 v = buf[lidx<INNER>(l, n, L, M)];
             if (P.blue) v = cmul(v, P.chirp[n]);
             v.x *= a.scale;

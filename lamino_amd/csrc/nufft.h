@@ -58,6 +58,7 @@ struct RowPlan {
     RowGeom rg;
     int Jn = 0;                        // rows of the half point set
     int dir = 1;                       // +1: iy non-decreasing in j, -1: non-increasing
+    int ylo = 0, ly = 0;               // the y planes the rows touch: [ylo, ylo + ly) mod nfy (ly 0: all)
     DevBuf wy;                         // float [Jn][W] row weights
     DevBuf iy;                         // int [Jn] unwrapped first y cell of each row
     DevBuf count;                      // unsigned [Jn][ntiles] points per (row, tile)

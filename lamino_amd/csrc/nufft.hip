@@ -1299,6 +1299,14 @@ RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
         }
     }
     rp->dir = iyh[rp->Jn - 1] >= iyh[0] ? 1 : -1;
+    {
+        const int lo = *std::min_element(iyh.begin(), iyh.end()), hi = *std::max_element(iyh.begin(), iyh.end());
+        const int len = hi + W - lo;
+        if (len < pl.nf[1]) {
+            rp->ylo = ((lo % pl.nf[1]) + pl.nf[1]) % pl.nf[1];
+            rp->ly = len;
+        }
+    }
     for (int j = 0; j + 1 < rp->Jn; ++j)
         LAM_REQUIRE(rp->dir * (iyh[j + 1] - iyh[j]) >= 0, LAM_ERR_STATE, "row first cells are not monotonic");
     rp->wy.alloc(sizeof(float) * wyh.size());
@@ -1545,7 +1553,25 @@ void crop_deapod(Ctx &cx, NufftPlan &pl, float *const out[3], float2 *cout, int 
 //  runs only on (z, y) mode lines and every pass reads only mode entries;
 //  inverse (type 1): only the mode region of the output is kept, so later
 //  passes run on fewer lines and every pass stores only mode entries.
-void fine_fft(Ctx &cx, NufftPlan &pl, int nc, int sign) {
+// The fine-grid y planes the three-component projectors of this grid use: the
+// rows' window on the row path, all planes otherwise.
+struct YWin {
+    int lo = 0, len = 0;
+};
+YWin y_window(Ctx &cx, Grid &g, NufftPlan &pl) {
+    YWin w;
+    if (!use_rows(g, pl)) return w;
+    const RowPlan &rp = get_rowplan(cx, g, pl);
+    w.lo = rp.ylo;
+    w.len = rp.ly;
+    return w;
+}
+
+// 3-D transform of the fine grid between the mode region and the full grid.
+// [ylo, ylo + ly) mod nfy (ly > 0): the only y planes the caller reads after
+// the forward transform / the only nonzero ones before the inverse (the rows
+// of a row-factorised projector); the z pass then runs over those planes only.
+void fine_fft(Ctx &cx, NufftPlan &pl, int nc, int sign, int ylo = 0, int ly = 0) {
     const int nfz = pl.nf[0], nfy = pl.nf[1], nfx = pl.nf[2];
     const int nz = (int)pl.n[0], ny = (int)pl.n[1], nx = (int)pl.n[2];
     const int hz = nz - nz / 2, hy = ny - ny / 2, hx = nx - nx / 2;   // head counts
@@ -1568,6 +1594,10 @@ void fine_fft(Ctx &cx, NufftPlan &pl, int nc, int sign) {
         p.c1 = nz; p.o1_split = hz; p.o1_gap = nfz - nz;
         p.c2 = 1; p.ci = nfx * nc;
         if (fwd) { p.nv = hy; p.nv2 = ny / 2; } else { p.ns = hy; p.ns2 = ny / 2; }
+        if (ly > 0) {
+            if (fwd) { p.st_lo = ylo; p.st_len = ly; }
+            else { p.ld_lo = ylo; p.ld_len = ly; }
+        }
         run_axis(cx, get_fft_plan(cx, nfy, sign), p);
     };
     auto z_pass = [&](bool fwd) {
@@ -1575,6 +1605,19 @@ void fine_fft(Ctx &cx, NufftPlan &pl, int nc, int sign) {
         p.in = p.out = f;
         p.is2 = p.os2 = sY; p.isi = p.osi = 1; p.isn = p.osn = sZ;
         p.c1 = 1; p.c2 = nfy; p.ci = nfx * nc;
+        if (ly > 0) {
+            // lines (y, x) for y in the window; line offsets must stay non-negative (the
+            // pass marks lines outside the set with -1), so a window wrapping past nfy
+            // is taken as [0, e2) followed by [ylo, nfy)
+            p.c2 = ly;
+            if (ylo + ly <= nfy) {
+                p.in = p.out = f + (long long)ylo * sY;
+            } else {
+                const int e2 = ylo + ly - nfy;
+                p.o2_split = e2;
+                p.o2_gap = ylo - e2;
+            }
+        }
         if (fwd) { p.nv = hz; p.nv2 = nz / 2; } else { p.ns = hz; p.ns2 = nz / 2; }
         run_axis(cx, get_fft_plan(cx, nfz, sign), p);
     };
@@ -1695,7 +1738,8 @@ void forward(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     cx.work[0].ensure(sizeof(float2) * M);
     float2 *cbuf = cx.work[0].as<float2>();
     deapod_pad(cx, pl, m, nullptr, 3);
-    fine_fft(cx, pl, 3, -1);
+    const YWin yw = y_window(cx, g, pl);
+    fine_fft(cx, pl, 3, -1, yw.lo, yw.len);
     launch_interp(cx, g, pl, cbuf, true);
     const float scale = 1.0f / (float)(g.R * g.C);
     if (paired_planes(g)) det_ifft2_real_pairs(cx, cbuf, proj, (int)g.P, (int)g.R, (int)g.C, +1, scale);
@@ -1713,7 +1757,8 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     if (paired) det_fft2_real_pairs(cx, proj, cbuf, (int)g.P, (int)g.R, (int)g.C, -1);
     else det_fft2(cx, proj, true, cbuf, false, (int)g.P, (int)g.R, (int)g.C, -1, 1.0f, cbuf);
     launch_spread(cx, g, pl, cbuf, true, paired);
-    fine_fft(cx, pl, 3, +1);
+    const YWin yw = y_window(cx, g, pl);
+    fine_fft(cx, pl, 3, +1, yw.lo, yw.len);
     crop_deapod(cx, pl, m, nullptr, 3, 1.0f / (float)(g.R * g.C), accumulate);
 }
 
@@ -1725,10 +1770,11 @@ void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const f
     cx.work[0].ensure(sizeof(float2) * M);
     float2 *cbuf = cx.work[0].as<float2>();
     deapod_pad(cx, pl, x, nullptr, 3);
-    fine_fft(cx, pl, 3, -1);
+    const YWin yw = y_window(cx, g, pl);
+    fine_fft(cx, pl, 3, -1, yw.lo, yw.len);
     launch_interp(cx, g, pl, cbuf, true);
     launch_spread(cx, g, pl, cbuf, true);
-    fine_fft(cx, pl, 3, +1);
+    fine_fft(cx, pl, 3, +1, yw.lo, yw.len);
     crop_deapod(cx, pl, y, nullptr, 3, 1.0f / (float)(g.R * g.C), false);
 }
 
