@@ -51,6 +51,7 @@ struct RowGeom {
     unsigned ntiles = 0;
     int HBz = 0, HBx = 0;      // box extents Tz + W - 1, Tx + W - 1
     int NH = 0;                // halo cells per tile: (W-1) Tx + Tz (W-1) + (W-1)^2
+    int XB = 1;                // spreader x block (divides Tx)
     FastDiv fbx;               // HBx
 };
 

@@ -837,12 +837,19 @@ __global__ __launch_bounds__(256) void k_rows_interp(const float2 *__restrict__ 
 // slab and the rest to the tile's halo record, both with plain stores.
 constexpr int kStageRows = 64;
 
+// x blocks: points whose first cells share z and lie in one block of B x cells
+// are summed into the same registers; the wave's lanes then cover W rows of
+// W + B - 1 cells.  B = 64 / W - W + 1 for one lane pass (5 at W = 6), 1 when
+// W^2 needs more than one.
+constexpr int row_xblock_max(int W) { return W * W <= 64 ? 64 / W - W + 1 : 1; }
+
 template <int W>
 struct RowStage {
-    static constexpr int VQ = 2;             // [0] box cell, [1] unused, values, wz[W], wx[W]
+    static constexpr int WB = W + row_xblock_max(W) - 1;   // x weights per record, shifted into place
+    static constexpr int VQ = 2;             // [0] box cell of (z, block start), [1] unused, values, wz[W], wx[WB]
     static constexpr int WZ = VQ + 6;
     static constexpr int WX = WZ + W;
-    static constexpr int SW = (WX + W + 3) / 4 * 4;
+    static constexpr int SW = (WX + WB + 3) / 4 * 4;
 };
 
 template <int W>
@@ -854,8 +861,10 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
                                                     const uint4 *__restrict__ chunks, int paired) {
     using SL = RowStage<W>;
     constexpr int SW = SL::SW;
-    constexpr int NST = W * W;
-    constexpr int NCH = (NST + 63) / 64;
+    constexpr int NCH = (W * W + 63) / 64;
+    const int XB = rg.XB;                 // x block (divides Tx; row_xblock_max(W) at most)
+    const int WBr = W + XB - 1;           // lane columns
+    const int NST = W * WBr;
     extern __shared__ float lds[];
     const uint4 ch = chunks[blockIdx.x];
     const Box2 bx = box2(ch.x, j0, rg);
@@ -873,7 +882,7 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
         const int l = lane + 64 * q;
         act[q] = l < NST;
         const int cc = act[q] ? l : 0;
-        const int a = cc / W, e = cc - (cc / W) * W;
+        const int a = cc / WBr, e = cc - a * WBr;
         loff[q] = a * HB + e;
         wzo[q] = SL::WZ + a;
         wxo[q] = SL::WX + e;
@@ -927,17 +936,23 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
             // can leave the box
             lz = min(max(lz, 0), rg.Tz - 1);
             lx = min(max(lx, 0), rg.Tx - 1);
+            const int x0 = lx - lx % XB, dx = lx - x0;   // block start; x0 + WBr <= HB as XB divides Tx
             float *sp = stg + lane * SW;
-            reinterpret_cast<int *>(sp)[0] = lz * HB + lx;
+            reinterpret_cast<int *>(sp)[0] = lz * HB + x0;
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 sp[SL::VQ + 2 * m] = vc[m].x;
                 sp[SL::VQ + 2 * m + 1] = vc[m].y;
             }
 #pragma unroll
-            for (int a = 0; a < W; ++a) {
-                sp[SL::WZ + a] = wz[a];
-                sp[SL::WX + a] = wx[a];
+            for (int a = 0; a < W; ++a) sp[SL::WZ + a] = wz[a];
+            // the x weights shifted by dx into the block's WB columns, zero elsewhere
+#pragma unroll
+            for (int e = 0; e < SL::WB; ++e) {
+                float v = 0.f;
+#pragma unroll
+                for (int m = 0; m < W; ++m) v = (e - dx == m) ? wx[m] : v;
+                sp[SL::WX + e] = v;
             }
         }
         __syncthreads();
@@ -1223,11 +1238,14 @@ long long half_rows(const Grid &g) {
     return (H - 1) / g.C + 1;
 }
 
-// a tile edge that divides n, at least w (the halo reaches one tile only), closest to the target
-int row_tile(int n, int w) {
+// a tile edge that divides n, is a multiple of mult, at least w (the halo reaches one tile
+// only), closest to the target; 0 if there is none
+int row_tile(int n, int w, int mult = 1) {
     int best = 0;
     for (int d = w; d <= std::min(n, 48); ++d)
-        if (n % d == 0 && (best == 0 || std::abs(d - kTileRowsTarget) < std::abs(best - kTileRowsTarget))) best = d;
+        if (n % d == 0 && d % mult == 0 &&
+            (best == 0 || std::abs(d - kTileRowsTarget) < std::abs(best - kTileRowsTarget)))
+            best = d;
     return best;
 }
 
@@ -1235,7 +1253,10 @@ RowGeom row_geom(const NufftPlan &pl) {
     RowGeom rg;
     const int W = pl.w;
     rg.Tz = row_tile(pl.nf[0], W);
-    rg.Tx = row_tile(pl.nf[2], W);
+    // the spreader's x block must divide Tx: the largest block that has such a tile
+    for (rg.XB = row_xblock_max(W); rg.XB >= 1; --rg.XB)
+        if ((rg.Tx = row_tile(pl.nf[2], W, rg.XB)) != 0) break;
+    if (rg.XB < 1) rg.XB = 1;
     if (!rg.Tz || !rg.Tx) return rg;
     rg.ntz = pl.nf[0] / rg.Tz;
     rg.ntx = pl.nf[2] / rg.Tx;
@@ -1260,7 +1281,7 @@ bool use_rows(const Grid &g, const NufftPlan &pl) {
 }
 
 size_t rows_spread_lds(const RowGeom &rg, int w) {
-    const size_t sw = (size_t)(2 + 6 + 2 * w + 3) / 4 * 4;   // RowStage<w>::SW
+    const size_t sw = (size_t)(2 + 6 + w + (w + row_xblock_max(w) - 1) + 3) / 4 * 4;   // RowStage<w>::SW
     return (size_t)kStageRows * sw * 4 + 3 * (size_t)rg.HBz * rg.HBx * 8;
 }
 size_t rows_interp_lds(const RowGeom &rg) { return 3 * (size_t)rg.HBz * rg.HBx * 8; }
