@@ -309,6 +309,19 @@ void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float 
                    float2 *work) {
     const int n1 = (int)d[0], n2 = (int)d[1], n3 = (int)d[2];
     const long long n = d[0] * d[1] * d[2];
+    const long long plane = (long long)n2 * n3;
+    // Two real slices go through one complex transform (the filter is real and even, so
+    // the two stay apart): the slices of in0 and in1, or, for one component, slice s
+    // and slice s + ceil(n1/2) of in0 (an odd n1 leaves the middle slice alone).
+    int planes = n1, paired = 1 << 30;
+    const float *i1 = in1;
+    float *o1 = out1;
+    if (!in1 && n1 > 1) {
+        planes = (n1 + 1) / 2;
+        paired = n1 - planes;
+        i1 = in0 + planes * plane;
+        o1 = out0 + planes * plane;
+    }
     float2 *a = work;
     if (!a) {
         cx.work[2].ensure(sizeof(float2) * (size_t)n);
@@ -317,39 +330,43 @@ void precond_apply(Ctx &cx, const long long d[3], const float *in0, const float 
     // rows (n3), real or real-pair in
     {
         AxisPass p;
-        p.in = in0; p.in2 = in1;
-        p.ld_mode = in1 ? LD_REAL_PAIR : LD_REAL;
-        p.is1 = (long long)n2 * n3; p.is2 = n3; p.isn = 1;
-        p.out = a; p.os1 = (long long)n2 * n3; p.os2 = n3; p.osn = 1;
-        p.c1 = n1; p.c2 = n2; p.ci = 1;
+        p.in = in0; p.in2 = i1;
+        p.ld_mode = i1 ? LD_REAL_PAIR : LD_REAL;
+        p.c1_in2 = paired;
+        p.is1 = plane; p.is2 = n3; p.isn = 1;
+        p.out = a; p.os1 = plane; p.os2 = n3; p.osn = 1;
+        p.c1 = planes; p.c2 = n2; p.ci = 1;
         run_axis(cx, get_fft_plan(cx, n3, -1), p);
     }
     {
         AxisPass p;
         p.in = a; p.out = a;
-        p.is1 = p.os1 = (long long)n2 * n3; p.isi = p.osi = 1; p.isn = p.osn = n3;
-        p.c1 = n1; p.c2 = 1; p.ci = n3;
+        p.is1 = p.os1 = plane; p.isi = p.osi = 1; p.isn = p.osn = n3;
+        p.c1 = planes; p.c2 = 1; p.ci = n3;
         run_axis(cx, get_fft_plan(cx, n2, -1), p);
     }
-    hipLaunchKernelGGL(k_ramp_filter, dim3(vgrid(n)), dim3(256), 0, cx.stream, a, (float2 *)nullptr, n1, n2, n3);
+    hipLaunchKernelGGL(k_ramp_filter, dim3(vgrid((long long)planes * plane)), dim3(256), 0, cx.stream, a,
+                       (float2 *)nullptr, planes, n2, n3);
     check_launch();
     {
         AxisPass p;
         p.in = a; p.out = a;
-        p.is1 = p.os1 = (long long)n2 * n3; p.isi = p.osi = 1; p.isn = p.osn = n3;
-        p.c1 = n1; p.c2 = 1; p.ci = n3;
+        p.is1 = p.os1 = plane; p.isi = p.osi = 1; p.isn = p.osn = n3;
+        p.c1 = planes; p.c2 = 1; p.ci = n3;
         run_axis(cx, get_fft_plan(cx, n2, +1), p);
     }
     {
         AxisPass p;
-        p.in = a; p.is1 = (long long)n2 * n3; p.is2 = n3; p.isn = 1;
-        p.out = out0; p.out2 = out1;
-        p.st_mode = out1 ? ST_REAL_PAIR : ST_REAL;
-        p.os1 = (long long)n2 * n3; p.os2 = n3; p.osn = 1;
-        p.c1 = n1; p.c2 = n2; p.ci = 1;
+        p.in = a; p.is1 = plane; p.is2 = n3; p.isn = 1;
+        p.out = out0; p.out2 = o1;
+        p.st_mode = o1 ? ST_REAL_PAIR : ST_REAL;
+        p.c1_out2 = paired;
+        p.os1 = plane; p.os2 = n3; p.osn = 1;
+        p.c1 = planes; p.c2 = n2; p.ci = 1;
         p.scale = (float)(1.0 / ((double)n2 * (double)n3));
         run_axis(cx, get_fft_plan(cx, n3, +1), p);
     }
+    (void)n;
 }
 
 void axpby3(Ctx &cx, long long n, float a, CVol3 x, float b, Vol3 y) {

@@ -43,7 +43,7 @@ def test_fft2_shifted_matches_reference_composition(ctx, orc, shape):
         assert rel(la.fft2(x, sign, shifted=True), want) < 5e-6
 
 
-@pytest.mark.parametrize("dims", [(5, 7, 9), (6, 8, 10), (4, 21, 89), (3, 43, 179)])
+@pytest.mark.parametrize("dims", [(5, 7, 9), (6, 8, 10), (4, 21, 89), (3, 43, 179), (1, 7, 9)])
 def test_precond_matches_oracle(ctx, orc, dims):
     import lamino_amd as la
 
