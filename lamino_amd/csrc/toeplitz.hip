@@ -27,33 +27,89 @@ __device__ __forceinline__ double dirichlet(double v, int n) {
     return sinpi(v) / den;
 }
 
-__global__ __launch_bounds__(256) void k_toep_build(float2 *__restrict__ K, const Geo *__restrict__ geo, int P, int R,
-                                                    int C, int n1, int n2, int n3, int L1, int L2, int L3,
-                                                    double scale) {
-    const int e3 = 2 * n3 - 1, e2 = 2 * n2 - 1, e1 = 2 * n1 - 1;
-    const long long total = (long long)e1 * e2 * e3;
+// Rotations that step the D_R arguments pi dv and pi dv / R of one projection
+// by one dy: (cos, sin) of pi vy and of pi vy / R.
+struct DyStep {
+    double c1, s1, c2, s2;
+};
+
+constexpr int kToepDy = 8;   // dy offsets per thread
+
+// Closed-form kernel T_ik(D) = scale sum_p d_i d_k D_C(D.u_p) D_R(D.v_p) on the offsets
+// D in [-(n-1), n-1]^3, added at D mod L (6 unique (i, k) pairs in 3 complex slots).
+// T is even in D (so are the Dirichlet kernels): only dx >= 0 is computed and each
+// value is added at D and -D; on the dx = 0 plane the half dz > 0 or (dz = 0, dy > 0)
+// is mirrored and the centre added once, so every offset is written exactly once.
+// D_C(D.u) does not depend on dy, and D.v is linear in dy: a thread takes kToepDy
+// consecutive dy, evaluates D_C once per projection and steps the sines of D_R by
+// rotation (error ~kToepDy ulp).
+__global__ __launch_bounds__(256) void k_toep_build(float2 *__restrict__ K, const Geo *__restrict__ geo,
+                                                    const DyStep *__restrict__ step, int P, int R, int C, int n1,
+                                                    int n2, int n3, int L1, int L2, int L3, double scale) {
+    const int e2 = 2 * n2 - 1;
+    const int nyc = (e2 + kToepDy - 1) / kToepDy;
+    const long long total = (long long)(2 * n1 - 1) * n3 * nyc;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int dx = (int)(id % e3) - (n3 - 1);
-        const int dy = (int)((id / e3) % e2) - (n2 - 1);
-        const int dz = (int)(id / ((long long)e3 * e2)) - (n1 - 1);
-        double a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
+        const int yc = (int)(id % nyc);
+        const int dx = (int)((id / nyc) % n3);
+        const int dz = (int)(id / ((long long)nyc * n3)) - (n1 - 1);
+        const int dy0 = -(n2 - 1) + yc * kToepDy;
+        const int ny = min(kToepDy, (n2 - 1) - dy0 + 1);
+        if (dx == 0 && dz < 0) continue;                                  // mirrored from dz > 0
+        if (dx == 0 && dz == 0 && dy0 + ny - 1 < 0) continue;            // mirrored from dy > 0
+        double acc[kToepDy][6];
+#pragma unroll
+        for (int k = 0; k < kToepDy; ++k)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[k][q] = 0.0;
         for (int p = 0; p < P; ++p) {
             const Geo g = geo[p];
-            const double du = dx * g.ux + dz * g.uz;
-            const double dv = dx * g.vx + dy * g.vy + dz * g.vz;
-            const double D = dirichlet(du, C) * dirichlet(dv, R);
-            const double e0 = g.d0 * D, e1v = g.d1 * D, e2v = g.d2 * D;
-            a00 += g.d0 * e0; a01 += g.d0 * e1v; a02 += g.d0 * e2v;
-            a11 += g.d1 * e1v; a12 += g.d1 * e2v; a22 += g.d2 * e2v;
+            const DyStep st = step[p];
+            const double Dc = dirichlet(dx * g.ux + dz * g.uz, C);
+            const double dv0 = dx * g.vx + dy0 * g.vy + dz * g.vz;
+            double s1, c1, s2, c2;
+            sincospi(dv0, &s1, &c1);
+            sincospi(dv0 / (double)R, &s2, &c2);
+#pragma unroll
+            for (int k = 0; k < kToepDy; ++k) {
+                if (k < ny) {
+                    double Dr;
+                    if (fabs(s2) < 1e-10) {
+                        const long long m = llrint((dv0 + k * g.vy) / (double)R);
+                        Dr = ((m * (long long)(R - 1)) & 1LL) ? -(double)R : (double)R;
+                    } else {
+                        Dr = s1 / s2;
+                    }
+                    const double D = Dc * Dr;
+                    const double e0 = g.d0 * D, e1v = g.d1 * D, e2v = g.d2 * D;
+                    acc[k][0] += g.d0 * e0; acc[k][1] += g.d0 * e1v; acc[k][2] += g.d0 * e2v;
+                    acc[k][3] += g.d1 * e1v; acc[k][4] += g.d1 * e2v; acc[k][5] += g.d2 * e2v;
+                    const double t1 = c1 * st.c1 - s1 * st.s1, t2 = c2 * st.c2 - s2 * st.s2;
+                    s1 = s1 * st.c1 + c1 * st.s1;
+                    s2 = s2 * st.c2 + c2 * st.s2;
+                    c1 = t1;
+                    c2 = t2;
+                }
+            }
         }
-        const int pz = dz < 0 ? dz + L1 : dz, py = dy < 0 ? dy + L2 : dy, px = dx < 0 ? dx + L3 : dx;
-        const long long pos = (((long long)pz * L2 + py) * L3 + px) * 3;
-        float2 k0 = K[pos], k1 = K[pos + 1], k2 = K[pos + 2];
-        k0.x += (float)(a00 * scale); k0.y += (float)(a01 * scale);
-        k1.x += (float)(a02 * scale); k1.y += (float)(a11 * scale);
-        k2.x += (float)(a12 * scale); k2.y += (float)(a22 * scale);
-        K[pos] = k0; K[pos + 1] = k1; K[pos + 2] = k2;
+        auto add = [&](int z, int y, int x, const double *a) {
+            const int pz = z < 0 ? z + L1 : z, py = y < 0 ? y + L2 : y, px = x < 0 ? x + L3 : x;
+            const long long pos = (((long long)pz * L2 + py) * L3 + px) * 3;
+            float2 k0 = K[pos], k1 = K[pos + 1], k2 = K[pos + 2];
+            k0.x += (float)(a[0] * scale); k0.y += (float)(a[1] * scale);
+            k1.x += (float)(a[2] * scale); k1.y += (float)(a[3] * scale);
+            k2.x += (float)(a[4] * scale); k2.y += (float)(a[5] * scale);
+            K[pos] = k0; K[pos + 1] = k1; K[pos + 2] = k2;
+        };
+#pragma unroll
+        for (int k = 0; k < kToepDy; ++k) {
+            if (k >= ny) break;
+            const int dy = dy0 + k;
+            if (dx == 0 && dz == 0 && dy < 0) continue;
+            add(dz, dy, dx, acc[k]);
+            if (dx != 0 || dz != 0 || dy != 0) add(-dz, -dy, -dx, acc[k]);
+        }
     }
 }
 
@@ -256,7 +312,6 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
     const size_t kbytes = sizeof(float2) * 3 * (size_t)Lt;
     t->khat.alloc(kbytes);
     LAM_HIP(hipMemsetAsync(t->khat.ptr, 0, kbytes, cx.stream));
-    const long long e = (2 * dims[0] - 1) * (2 * dims[1] - 1) * (2 * dims[2] - 1);
     for (int g = 0; g < ngrids; ++g) {
         Grid &gr = *grids[g];
         float2 *dst = t->khat.as<float2>();
@@ -266,11 +321,21 @@ std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const
             LAM_HIP(hipMemsetAsync(dst, 0, kbytes, cx.stream));
         }
         const double scale = 1.0 / ((double)gr.R * (double)gr.C * (double)Lt);
+        std::vector<DyStep> steps(gr.P);
+        for (long long p = 0; p < gr.P; ++p) {
+            const double vy = gr.geo_host[p].vy, pi = 3.14159265358979323846;
+            steps[p] = {std::cos(pi * vy), std::sin(pi * vy), std::cos(pi * vy / (double)gr.R),
+                        std::sin(pi * vy / (double)gr.R)};
+        }
+        DevBuf dsteps(sizeof(DyStep) * steps.size());
+        upload_now(cx, dsteps.ptr, steps.data(), sizeof(DyStep) * steps.size());
         {
             TimedScope ts(cx, "toep_build");
-            hipLaunchKernelGGL(k_toep_build, dim3(lgrid(e)), dim3(256), 0, cx.stream, dst, gr.geo.as<Geo>(),
-                               (int)gr.P, (int)gr.R, (int)gr.C, (int)dims[0], (int)dims[1], (int)dims[2], L1, L2, L3,
-                               scale);
+            const long long nthreads =
+                (2 * dims[0] - 1) * dims[2] * ((2 * dims[1] - 1 + kToepDy - 1) / kToepDy);
+            hipLaunchKernelGGL(k_toep_build, dim3(lgrid(nthreads)), dim3(256), 0, cx.stream, dst, gr.geo.as<Geo>(),
+                               dsteps.as<DyStep>(), (int)gr.P, (int)gr.R, (int)gr.C, (int)dims[0], (int)dims[1],
+                               (int)dims[2], L1, L2, L3, scale);
             check_launch();
         }
         if (per_grid) {
