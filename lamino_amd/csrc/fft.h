@@ -4,10 +4,17 @@
 // (o1, o2, i) lives at  o1*s1 + o2*s2 + i*si + n*sn  (separate strides for the
 // input and the output, in elements of their own type).  One workgroup owns
 // L lines: consecutive i when ci > 1 (so that the loads of a row of the tile
-// are contiguous), otherwise consecutive o2.  The transform is a mixed-radix
-// Stockham autosort (radices 2,3,4,5) or, for lengths with another prime
-// factor, Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length;
-// both run entirely in LDS between one load and one store.
+// are contiguous), otherwise consecutive o2.  The transform is one of
+//   * a mixed-radix Stockham autosort (radices 2,3,4,5) for 2,3,5-smooth n;
+//   * for n = S * Q with S 2,3,5-smooth and Q > 5 a prime whose Q - 1 is
+//     2,3,5-smooth (305 = 5 * 61, 615 = 15 * 41, 7, 11, 13, ...): the
+//     Good-Thomas prime-factor split into S x Q (no twiddles between them),
+//     Stockham over S, and Rader's algorithm over Q, which turns a length-Q
+//     DFT into a cyclic convolution of length Q - 1 done with two smooth
+//     Stockham FFTs;
+//   * otherwise Bluestein's chirp-z algorithm on a 2,3,5-smooth inner length
+//     m >= 2n - 1;
+// all entirely in LDS between one load and one store.
 //
 // Fused at the load:  rotation (fftshift), zero padding beyond nv, real,
 // real-pair (re from in, im from in2) or complex-pair (in + i in2) input.
@@ -24,11 +31,17 @@ enum { ST_COMPLEX = 0, ST_REAL = 1, ST_REAL_PAIR = 2, ST_IMAG = 3 };
 struct FftPlan {
     int n = 0, sign = -1, m = 0;
     bool blue = false;
+    // prime-factor + Rader plan (n = S * Q): rad[0, nrad_s) factor S, rad[nrad_s, nrad) factor Q - 1
+    bool rader = false;
+    int S = 1, Q = 1, nrad_s = 0;
     int nrad = 0;
     int rad[24] = {};
-    DevBuf roots;  // m entries exp(-2 pi i k / m)
-    DevBuf chirp;  // n entries exp(sign pi i k^2 / n)        (Bluestein)
-    DevBuf bhat;   // m entries FFT_m(conj chirp, wrapped) / m (Bluestein)
+    DevBuf roots;   // m entries exp(-2 pi i k / m)   (Rader: S entries)
+    DevBuf chirp;   // n entries exp(sign pi i k^2 / n)        (Bluestein)
+    DevBuf bhat;    // m entries FFT_m(conj chirp, wrapped) / m (Bluestein);
+                    // Rader: Q - 1 entries FFT_{Q-1}(b) / (Q - 1), b_j = exp(sign 2 pi i g^j / Q)
+    DevBuf roots_q; // Rader: Q - 1 entries exp(-2 pi i k / (Q - 1))
+    DevBuf pos;     // Rader: 2n shorts, the LDS slot of input element k, then of output element k
 };
 
 struct AxisPass {

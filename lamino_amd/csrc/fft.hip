@@ -19,15 +19,32 @@ constexpr int kMaxLines = 128;
 // Every index split by a launch-time divisor goes through FastDiv: a plain
 // integer division costs ~30 VALU instructions and the passes do several per
 // element.
+// A set of nv equal sub-lines of the tile (prime-factor plans): sub-line v is
+// column c of tile line l (INNER: v = c * L + l, else v = l * C + c), and its
+// element t sits at position off + c * coff + t * str of line l.
+struct SubLines {
+    int nv = 0, C = 1, off = 0, coff = 0, str = 1;
+    FastDiv fnv, fC;
+};
+
+enum { KIND_STOCKHAM = 0, KIND_BLUESTEIN = 1, KIND_RADER = 2 };
+
 struct KParams {
     AxisPass a;
-    int n, m, blue, nrad, sign;
+    int n, m, kind, nrad, sign;
     int L, Li, Lo;  // lines per workgroup, L = Li * Lo
     int rad[24];
     const float2 *roots, *chirp, *bhat;
     FastDiv fL, fN, fM, fLi, fPad;        // tile lines, length, inner length, lines per o2 row, M - N
     FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (M / R), span Ns
     int isn, osn;                         // element strides along the transform (fit 32 bits)
+    // Rader plans: n = S * Q; the S-point transforms run over the Q columns of
+    // each line (sub_s), the Q - 1 point convolutions over its S rows (sub_q)
+    int S, Q, nrad_s;
+    const float2 *roots_q;
+    const short *pos_in, *pos_out;
+    SubLines sub_s, sub_q;
+    FastDiv fQ1;                          // Q - 1
 };
 
 // LDS position of element n of tile line l: n-major when the tile runs along
@@ -35,6 +52,29 @@ struct KParams {
 template <int INNER>
 __device__ __forceinline__ int lidx(int l, int n, int L, int M) {
     return INNER ? n * L + l : l * M + n;
+}
+
+// Element e of a stage over nv (sub-)lines of nbf butterflies each: its
+// butterfly j and the LDS base and step of its (sub-)line.  Lanes run along
+// the lines when the tile is n-major (INNER), along the butterflies otherwise.
+template <int INNER, bool SUB>
+__device__ __forceinline__ void locate(const KParams &P, const SubLines &g, const FastDiv &fNbf, int nbf, int e,
+                                       int &j, int &base, int &step) {
+    const int nv = SUB ? g.nv : P.L;
+    int v;
+    if (INNER) { j = (int)(SUB ? g.fnv : P.fL).div((unsigned)e); v = e - j * nv; }
+    else { v = (int)fNbf.div((unsigned)e); j = e - v * nbf; }
+    if (!SUB) {
+        base = INNER ? v : v * P.m;
+        step = INNER ? P.L : 1;
+        return;
+    }
+    int l, c;
+    if (INNER) { c = (int)P.fL.div((unsigned)v); l = v - c * P.L; }
+    else { l = (int)g.fC.div((unsigned)v); c = v - l * g.C; }
+    const int p0 = g.off + c * g.coff;
+    base = INNER ? p0 * P.L + l : l * P.m + p0;
+    step = INNER ? g.str * P.L : g.str;
 }
 
 __device__ __forceinline__ float2 twiddle(const float2 *roots, int idx, int sign) {
@@ -86,28 +126,24 @@ __device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, in
     }
 }
 
-template <int R, int INNER>
-__device__ __forceinline__ void stage(float2 *__restrict__ buf, int L, int M, int Ns, int sign,
-                                      const float2 *roots, const FastDiv &fL, const FastDiv &fNbf,
-                                      const FastDiv &fNs) {
+// One radix-R Stockham stage over the tile's lines (SUB = false: L lines of
+// length m) or over a sub-line set (length len).
+template <int R, int INNER, bool SUB>
+__device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P, const SubLines &g, int len, int Ns,
+                                      int sign, const float2 *roots, const FastDiv &fNbf, const FastDiv &fNs) {
     constexpr int MAXB = (kLdsComplex / R + kThreads - 1) / kThreads;
-    const int nbf = M / R;
-    const int tot = L * nbf;
-    const int tstep = M / (Ns * R);
+    const int nbf = len / R;
+    const int tot = (SUB ? g.nv : P.L) * nbf;
+    const int tstep = len / (Ns * R);
     float2 v[MAXB][R];
-    // element e of the stage is butterfly j of tile line l
-    auto split = [&](int e, int &l, int &j) {
-        if (INNER) { j = (int)fL.div((unsigned)e); l = e - j * L; }
-        else { l = (int)fNbf.div((unsigned)e); j = e - l * nbf; }
-    };
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int e = threadIdx.x + b * kThreads;
         if (e < tot) {
-            int l, j;
-            split(e, l, j);
+            int j, base, step;
+            locate<INNER, SUB>(P, g, fNbf, nbf, e, j, base, step);
 #pragma unroll
-            for (int t = 0; t < R; ++t) v[b][t] = buf[lidx<INNER>(l, j + t * nbf, L, M)];
+            for (int t = 0; t < R; ++t) v[b][t] = buf[base + (j + t * nbf) * step];
         }
     }
     __syncthreads();
@@ -115,36 +151,65 @@ __device__ __forceinline__ void stage(float2 *__restrict__ buf, int L, int M, in
     for (int b = 0; b < MAXB; ++b) {
         const int e = threadIdx.x + b * kThreads;
         if (e < tot) {
-            int l, j;
-            split(e, l, j);
+            int j, base, step;
+            locate<INNER, SUB>(P, g, fNbf, nbf, e, j, base, step);
             const int jq = (int)fNs.div((unsigned)j);
             const int k = j - jq * Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], twiddle(roots, t * k * tstep, sign));
             }
-            dft<R>(v[b], sign, roots, M / R);
-            const int base = jq * Ns * R + k;
+            dft<R>(v[b], sign, roots, len / R);
+            const int o = jq * Ns * R + k;
 #pragma unroll
-            for (int t = 0; t < R; ++t) buf[lidx<INNER>(l, base + t * Ns, L, M)] = v[b][t];
+            for (int t = 0; t < R; ++t) buf[base + (o + t * Ns) * step] = v[b][t];
         }
     }
     __syncthreads();
 }
 
-template <int INNER>
-__device__ void stages(float2 *buf, const KParams &P, int sign) {
+// Stockham stages rad[first, first + count) over (sub-)lines of length len
+// whose root table holds exp(-2 pi i k / len).
+template <int INNER, bool SUB>
+__device__ void stages(float2 *buf, const KParams &P, const SubLines &g, int len, int first, int count, int sign,
+                       const float2 *roots) {
     int Ns = 1;
-    for (int s = 0; s < P.nrad; ++s) {
+    for (int s = first; s < first + count; ++s) {
         const int r = P.rad[s];
         switch (r) {
-            case 2: stage<2, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
-            case 3: stage<3, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
-            case 4: stage<4, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
-            default: stage<5, INNER>(buf, P.L, P.m, Ns, sign, P.roots, P.fL, P.fNbf[s], P.fNs[s]); break;
+            case 2: stage<2, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
+            case 3: stage<3, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
+            case 4: stage<4, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
+            default: stage<5, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
         }
         Ns *= r;
     }
+}
+
+// Rader's step between the two convolution FFTs.  Row c of line l holds y0 (the
+// row's DFT input at index 0) at slot 0 and A = FFT_{Q-1}(a) at slots 1..Q-1,
+// a_p = y[g^-p].  Afterwards slot 0 holds X[0] = y0 + sum_p a_p = y0 + A[0] and
+// slots 1.. hold A * bhat with y0 added to the zero frequency, so that the
+// unnormalised inverse FFT gives X[g^q] = y0 + (a * b)_q at slot 1 + q.
+template <int INNER>
+__device__ __forceinline__ void rader_pointwise(float2 *buf, const KParams &P) {
+    const SubLines &g = P.sub_q;
+    const int q1 = P.Q - 1;
+    for (int e = threadIdx.x; e < g.nv * q1; e += blockDim.x) {
+        int j, base, step;
+        locate<INNER, true>(P, g, P.fQ1, q1, e, j, base, step);
+        const int id = base + j * step;
+        const float2 A = buf[id];
+        float2 c = cmul(A, P.bhat[j]);
+        if (j == 0) {
+            const int id0 = base - step;  // slot 0 of the row
+            const float2 y0 = buf[id0];
+            buf[id0] = cadd(y0, A);
+            c = cadd(c, y0);
+        }
+        buf[id] = c;
+    }
+    __syncthreads();
 }
 
 __device__ __forceinline__ int remap(int o, int split, int gap) { return o < split ? o : o + gap; }
@@ -218,10 +283,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
                 }
             }
         }
-        if (P.blue) v = cmul(v, P.chirp[n]);
-        buf[lidx<INNER>(l, n, L, M)] = v;
+        if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[n]);
+        buf[lidx<INNER>(l, P.kind == KIND_RADER ? (int)P.pos_in[n] : n, L, M)] = v;
     }
-    if (P.blue) {
+    if (P.kind == KIND_BLUESTEIN) {
         const int pad = M - N;
         for (int e = threadIdx.x; e < L * pad; e += blockDim.x) {
             int l, n;
@@ -232,10 +297,16 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     }
     __syncthreads();
 
-    if (!P.blue) {
-        stages<INNER>(buf, P, P.sign);
+    const SubLines whole;
+    if (P.kind == KIND_STOCKHAM) {
+        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, P.sign, P.roots);
+    } else if (P.kind == KIND_RADER) {
+        if (P.S > 1) stages<INNER, true>(buf, P, P.sub_s, P.S, 0, P.nrad_s, P.sign, P.roots);
+        stages<INNER, true>(buf, P, P.sub_q, P.Q - 1, P.nrad_s, P.nrad - P.nrad_s, -1, P.roots_q);
+        rader_pointwise<INNER>(buf, P);
+        stages<INNER, true>(buf, P, P.sub_q, P.Q - 1, P.nrad_s, P.nrad - P.nrad_s, +1, P.roots_q);
     } else {
-        stages<INNER>(buf, P, -1);
+        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, -1, P.roots);
         for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
             int l, n;
             if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
@@ -244,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
             buf[id] = cmul(buf[id], P.bhat[n]);
         }
         __syncthreads();
-        stages<INNER>(buf, P, +1);
+        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, +1, P.roots);
     }
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
@@ -255,8 +326,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
         if (mm >= N) mm -= N;
         const long long lb = base_out[l];
         if (lb >= 0 && (mm < ns || mm >= N - a.ns2) && in_window(mm, a.st_lo, a.st_len, N)) {
-            float2 v = buf[lidx<INNER>(l, n, L, M)];
-            if (P.blue) v = cmul(v, P.chirp[n]);
+            float2 v = buf[lidx<INNER>(l, P.kind == KIND_RADER ? (int)P.pos_out[n] : n, L, M)];
+            if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[n]);
             v.x *= a.scale;
             v.y *= a.scale;
             const long long off = lb + (long long)mm * P.osn;
@@ -291,6 +362,109 @@ bool smooth235(int n) {
     return n == 1;
 }
 
+bool is_prime(int q) {
+    if (q < 2) return false;
+    for (int d = 2; d * d <= q; ++d)
+        if (q % d == 0) return false;
+    return true;
+}
+
+// n = S * Q with S 2,3,5-smooth and Q > 5 prime, Q - 1 2,3,5-smooth (largest such Q)
+bool rader_split(int n, int &S, int &Q) {
+    for (int q = n; q > 5; --q) {
+        if (n % q == 0 && is_prime(q) && smooth235(q - 1) && smooth235(n / q)) {
+            S = n / q;
+            Q = q;
+            return true;
+        }
+    }
+    return false;
+}
+
+int primitive_root(int q) {
+    for (int g = 2; g < q; ++g) {
+        long long x = 1;
+        int e = 1;
+        for (; e < q - 1; ++e) {
+            x = x * g % q;
+            if (x == 1) break;
+        }
+        if (e == q - 1) return g;
+    }
+    return 1;
+}
+
+int inverse_mod(int a, int m) {  // m > 1, gcd(a, m) = 1
+    for (int x = 1; x < m; ++x)
+        if ((long long)a * x % m == 1) return x;
+    return 0;
+}
+
+std::vector<float2> root_table(int m) {
+    const long double twopi = 6.283185307179586476925286766559L;
+    std::vector<float2> r(m);
+    for (int k = 0; k < m; ++k) {
+        long double ang = -twopi * (long double)k / (long double)m;
+        r[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    return r;
+}
+
+// Prime-factor (Good-Thomas) + Rader tables.  Input element k goes to row
+// n1 = k u1 mod S, column n2 = k u2 mod Q (u1 = Q^-1 mod S, u2 = S^-1 mod Q);
+// output element k comes from row k mod S, column k mod Q; then
+//   X[k] = sum_{n1,n2} x[n1,n2] w_S^{n1 k1} w_Q^{n2 k2}.
+// Within a row, column 0 stays at slot 0, input column g^-p goes to slot 1 + p
+// and output column g^q comes from slot 1 + q (g a primitive root mod Q).
+void build_rader(Ctx &cx, FftPlan &plan) {
+    const int n = plan.n, S = plan.S, Q = plan.Q, q1 = Q - 1;
+    const int g = primitive_root(Q);
+    std::vector<int> lg(Q, 0);  // discrete log base g
+    {
+        long long x = 1;
+        for (int e = 0; e < q1; ++e) { lg[(int)x] = e; x = x * g % Q; }
+    }
+    const int u1 = S > 1 ? inverse_mod(Q % S, S) : 0, u2 = inverse_mod(S % Q, Q);
+    std::vector<short> pos(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const int n1 = S > 1 ? (int)((long long)k * u1 % S) : 0;
+        const int n2 = (int)((long long)k * u2 % Q);
+        pos[k] = (short)(n1 * Q + (n2 == 0 ? 0 : 1 + (q1 - lg[n2]) % q1));
+        const int k1 = k % S, k2 = k % Q;
+        pos[n + k] = (short)(k1 * Q + (k2 == 0 ? 0 : 1 + lg[k2]));
+    }
+    // bhat = FFT_{Q-1}(b) / (Q - 1), b_j = exp(sign 2 pi i g^j / Q), in double
+    const long double twopi = 6.283185307179586476925286766559L;
+    std::vector<std::complex<double>> b(q1);
+    {
+        long long x = 1;
+        for (int j = 0; j < q1; ++j) {
+            long double ang = (long double)plan.sign * twopi * (long double)x / (long double)Q;
+            b[j] = std::complex<double>((double)std::cos(ang), (double)std::sin(ang));
+            x = x * g % Q;
+        }
+    }
+    std::vector<float2> bh(q1);
+    for (int f = 0; f < q1; ++f) {
+        std::complex<double> acc = 0.0;
+        for (int j = 0; j < q1; ++j) {
+            long double ang = -twopi * (long double)((long long)j * f % q1) / (long double)q1;
+            acc += b[j] * std::complex<double>((double)std::cos(ang), (double)std::sin(ang));
+        }
+        acc /= (double)q1;
+        bh[f] = make_float2((float)acc.real(), (float)acc.imag());
+    }
+    const auto rs = root_table(S), rq = root_table(q1);
+    plan.roots.alloc(sizeof(float2) * S);
+    upload_now(cx, plan.roots.ptr, rs.data(), sizeof(float2) * S);
+    plan.roots_q.alloc(sizeof(float2) * q1);
+    upload_now(cx, plan.roots_q.ptr, rq.data(), sizeof(float2) * q1);
+    plan.bhat.alloc(sizeof(float2) * q1);
+    upload_now(cx, plan.bhat.ptr, bh.data(), sizeof(float2) * q1);
+    plan.pos.alloc(sizeof(short) * pos.size());
+    upload_now(cx, plan.pos.ptr, pos.data(), sizeof(short) * pos.size());
+}
+
 }  // namespace
 
 FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
@@ -302,6 +476,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
     plan->n = n;
     plan->sign = sign;
     int rad[24], nrad;
+    int S = 1, Q = 1;
     if (n == 1) {
         plan->m = 1;
         plan->nrad = 0;
@@ -309,6 +484,22 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         plan->m = n;
         plan->nrad = nrad;
         for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
+    } else if (n <= kLdsComplex && rader_split(n, S, Q)) {
+        plan->rader = true;
+        plan->m = n;
+        plan->S = S;
+        plan->Q = Q;
+        int rs[24], ns = 0, rq[24], nq = 0;
+        if (S > 1) factor_len(S, rs, ns);
+        factor_len(Q - 1, rq, nq);
+        plan->nrad_s = ns;
+        plan->nrad = ns + nq;
+        for (int i = 0; i < ns; ++i) plan->rad[i] = rs[i];
+        for (int i = 0; i < nq; ++i) plan->rad[ns + i] = rq[i];
+        build_rader(cx, *plan);
+        FftPlan &ref = *plan;
+        cx.fft_plans.emplace(key, std::move(plan));
+        return ref;
     } else {
         plan->blue = true;
         int m = 2 * n - 1;
@@ -374,7 +565,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.a = a;
     P.n = plan.n;
     P.m = plan.m;
-    P.blue = plan.blue ? 1 : 0;
+    P.kind = plan.rader ? KIND_RADER : plan.blue ? KIND_BLUESTEIN : KIND_STOCKHAM;
     P.nrad = plan.nrad;
     for (int i = 0; i < 24; ++i) P.rad[i] = plan.rad[i];
     P.sign = plan.sign;
@@ -403,13 +594,40 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.fM = FastDiv((unsigned)plan.m);
     P.fLi = FastDiv((unsigned)P.Li);
     P.fPad = FastDiv((unsigned)std::max(1, plan.m - plan.n));
-    {
+    if (plan.rader) {
+        // stages over S, then over Q - 1, each with its own spans
+        int Ns = 1;
+        for (int s = 0; s < plan.nrad; ++s) {
+            if (s == plan.nrad_s) Ns = 1;
+            const int len = s < plan.nrad_s ? plan.S : plan.Q - 1;
+            P.fNbf[s] = FastDiv((unsigned)(len / plan.rad[s]));
+            P.fNs[s] = FastDiv((unsigned)Ns);
+            Ns *= plan.rad[s];
+        }
+        P.S = plan.S;
+        P.Q = plan.Q;
+        P.nrad_s = plan.nrad_s;
+        P.roots_q = plan.roots_q.as<float2>();
+        P.pos_in = plan.pos.as<short>();
+        P.pos_out = P.pos_in + plan.n;
+        P.fQ1 = FastDiv((unsigned)(plan.Q - 1));
+        // S-point transforms: Q columns per line, element t of column c at t * Q + c
+        P.sub_s.nv = P.L * plan.Q; P.sub_s.C = plan.Q; P.sub_s.off = 0; P.sub_s.coff = 1; P.sub_s.str = plan.Q;
+        P.sub_s.fnv = FastDiv((unsigned)P.sub_s.nv); P.sub_s.fC = FastDiv((unsigned)plan.Q);
+        // Q - 1 point convolutions: S rows per line, element t of row c at c * Q + 1 + t
+        P.sub_q.nv = P.L * plan.S; P.sub_q.C = plan.S; P.sub_q.off = 1; P.sub_q.coff = plan.Q; P.sub_q.str = 1;
+        P.sub_q.fnv = FastDiv((unsigned)P.sub_q.nv); P.sub_q.fC = FastDiv((unsigned)plan.S);
+    } else {
         int Ns = 1;
         for (int s = 0; s < plan.nrad; ++s) {
             P.fNbf[s] = FastDiv((unsigned)(plan.m / plan.rad[s]));
             P.fNs[s] = FastDiv((unsigned)Ns);
             Ns *= plan.rad[s];
         }
+        P.S = P.Q = 1;
+        P.nrad_s = 0;
+        P.roots_q = nullptr;
+        P.pos_in = P.pos_out = nullptr;
     }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L;
