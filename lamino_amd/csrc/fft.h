@@ -5,7 +5,10 @@
 // input and the output, in elements of their own type).  One workgroup owns
 // L lines: consecutive i when ci > 1 (so that the loads of a row of the tile
 // are contiguous), otherwise consecutive o2.  The transform is one of
-//   * a mixed-radix Stockham autosort (radices 2,3,4,5) for 2,3,5-smooth n;
+//   * a mixed-radix Stockham autosort for 2,3,5-smooth n, each stage one
+//     register-resident radix of 2..12 (composites 6, 8, 9, 10, 12 are
+//     Cooley-Tukey splits with compile-time twiddles), as few stages as the
+//     radices allow (360 = 9 * 8 * 5, 90 = 10 * 9);
 //   * for n = S * Q with S 2,3,5-smooth and Q > 5 a prime whose Q - 1 is
 //     2,3,5-smooth (305 = 5 * 61, 615 = 15 * 41, 7, 11, 13, ...): the
 //     Good-Thomas prime-factor split into S x Q (no twiddles between them),

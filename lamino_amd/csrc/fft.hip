@@ -1,6 +1,9 @@
 // LDS-resident mixed-radix Stockham / Bluestein FFT passes for gfx950.
 #include <cmath>
+#include <algorithm>
 #include <complex>
+#include <cstdlib>
+#include <vector>
 
 #include "fastdiv.h"
 #include "fft.h"
@@ -9,16 +12,19 @@ namespace lam {
 
 namespace {
 
-constexpr int kThreads = 512;
+// Workgroup size; with register radices up to 12 a stage keeps most of the 256
+// threads busy (512 measured no faster).
+constexpr int kThreads = 256;
 // The workgroup's data tile: L * m <= 3840 complex (30 KiB) plus its line-base
 // tables (2 x L x 8 B <= 2 KiB).  Measured: small tiles with several workgroups
 // per CU hide the load/stage/store latency better than fewer, longer tiles.
 constexpr int kLdsComplex = 3840;
 constexpr int kMaxLines = 128;
 
-// Every index split by a launch-time divisor goes through FastDiv: a plain
-// integer division costs ~30 VALU instructions and the passes do several per
-// element.
+// Lines per tile along the inner (contiguous) index, LDS permitting: measured
+// 16 -> 48 took the 90-point passes of the benchmark grid from 94 to 76 us.
+constexpr int kInnerLines = 48;
+
 // A set of nv equal sub-lines of the tile (prime-factor plans): sub-line v is
 // column c of tile line l (INNER: v = c * L + l, else v = l * C + c), and its
 // element t sits at position off + c * coff + t * str of line l.
@@ -29,21 +35,30 @@ struct SubLines {
 
 enum { KIND_STOCKHAM = 0, KIND_BLUESTEIN = 1, KIND_RADER = 2 };
 
+// Every index split by a launch-time divisor goes through FastDiv: a plain
+// integer division costs ~30 VALU instructions and the passes do several per
+// element.
+//
+// The transform is a list of Stockham stages, each over a (sub-)line set:
+// st_g = 0: the tile's L lines of length m; 1: the S-point columns and 2: the
+// (Q - 1)-point rows of a Rader plan (roots_q).  Bluestein and Rader plans apply
+// their pointwise product after stage pw_after.
 struct KParams {
     AxisPass a;
-    int n, m, kind, nrad, sign;
+    int n, m, kind, sign;
     int L, Li, Lo;  // lines per workgroup, L = Li * Lo
-    int rad[24];
+    int nst, pw_after;
+    int rad[24], st_len[24], st_sign[24], st_g[24], st_ns[24];
     const float2 *roots, *chirp, *bhat;
     FastDiv fL, fN, fM, fLi, fPad;        // tile lines, length, inner length, lines per o2 row, M - N
-    FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (M / R), span Ns
+    FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (len / R), span Ns
     int isn, osn;                         // element strides along the transform (fit 32 bits)
     // Rader plans: n = S * Q; the S-point transforms run over the Q columns of
     // each line (sub_s), the Q - 1 point convolutions over its S rows (sub_q)
-    int S, Q, nrad_s;
+    int S, Q;
     const float2 *roots_q;
     const short *pos_in, *pos_out;
-    SubLines sub_s, sub_q;
+    SubLines whole, sub_s, sub_q;
     FastDiv fQ1;                          // Q - 1
 };
 
@@ -57,14 +72,14 @@ __device__ __forceinline__ int lidx(int l, int n, int L, int M) {
 // Element e of a stage over nv (sub-)lines of nbf butterflies each: its
 // butterfly j and the LDS base and step of its (sub-)line.  Lanes run along
 // the lines when the tile is n-major (INNER), along the butterflies otherwise.
-template <int INNER, bool SUB>
-__device__ __forceinline__ void locate(const KParams &P, const SubLines &g, const FastDiv &fNbf, int nbf, int e,
-                                       int &j, int &base, int &step) {
-    const int nv = SUB ? g.nv : P.L;
+template <int INNER>
+__device__ __forceinline__ void locate(const KParams &P, const SubLines &g, bool sub, const FastDiv &fNbf, int nbf,
+                                       int e, int &j, int &base, int &step) {
+    const int nv = g.nv;
     int v;
-    if (INNER) { j = (int)(SUB ? g.fnv : P.fL).div((unsigned)e); v = e - j * nv; }
+    if (INNER) { j = (int)g.fnv.div((unsigned)e); v = e - j * nv; }
     else { v = (int)fNbf.div((unsigned)e); j = e - v * nbf; }
-    if (!SUB) {
+    if (!sub) {
         base = INNER ? v : v * P.m;
         step = INNER ? P.L : 1;
         return;
@@ -83,15 +98,66 @@ __device__ __forceinline__ float2 twiddle(const float2 *roots, int idx, int sign
     return w;
 }
 
-// v'[s] = sum_t v[t] exp(sign 2 pi i t s / R)
-template <int R>
-__device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, int mstep) {
-    if constexpr (R == 2) {
+// Compile-time cos/sin (Taylor series on [-pi, pi], double) for the internal
+// twiddles of the composite radices; every use has constant arguments after
+// unrolling, so they fold into instruction literals.
+constexpr double kPi = 3.14159265358979323846;
+constexpr double cx_sin_series(double x) {
+    double term = x, sum = x;
+    for (int k = 1; k < 24; ++k) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cx_cos_series(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 24; ++k) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+// exp(SIGN 2 pi i m / R), argument reduced to (-pi, pi]
+template <int R, int SIGN>
+constexpr double tw_re(int m) {
+    m %= R;
+    if (2 * m > R) m -= R;
+    return cx_cos_series(2.0 * kPi * m / R);
+}
+template <int R, int SIGN>
+constexpr double tw_im(int m) {
+    m %= R;
+    if (2 * m > R) m -= R;
+    return SIGN * cx_sin_series(2.0 * kPi * m / R);
+}
+
+// Split of a composite radix R = A * B used by dft (0: a base radix)
+constexpr int radix_a(int R) {
+    switch (R) {
+        case 6: return 2;
+        case 8: return 2;
+        case 9: return 3;
+        case 10: return 2;
+        case 12: return 3;
+        default: return 0;
+    }
+}
+
+// v'[s] = sum_t v[t] exp(SIGN 2 pi i t s / R), in registers.  Radices 2..5 are
+// written out; a composite R = A * B runs B-point transforms over t = a + A b,
+// the twiddles exp(SIGN 2 pi i a kb / R), and A-point transforms over a, giving
+// output kb + B ka.
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(float2 *v) {
+    constexpr float sg = (float)SIGN;
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) {
         float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 3) {
-        const float s3 = 0.86602540378443864676f * (float)sign;
+        const float s3 = 0.86602540378443864676f * sg;
         float2 a = v[0], b = v[1], c = v[2];
         float2 t1 = cadd(b, c);
         float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
@@ -103,14 +169,14 @@ __device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, in
     } else if constexpr (R == 4) {
         float2 a = v[0], b = v[1], c = v[2], d = v[3];
         float2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
-        float2 is3 = make_float2(-(float)sign * s3.y, (float)sign * s3.x);  // sign * i * (b - d)
+        float2 is3 = make_float2(-sg * s3.y, sg * s3.x);  // SIGN * i * (b - d)
         v[0] = cadd(s0, s2);
         v[1] = cadd(s1, is3);
         v[2] = csub(s0, s2);
         v[3] = csub(s1, is3);
     } else if constexpr (R == 5) {
         const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-        const float s1 = 0.95105651629515357212f * (float)sign, s2 = 0.58778525229247312917f * (float)sign;
+        const float s1 = 0.95105651629515357212f * sg, s2 = 0.58778525229247312917f * sg;
         float2 a = v[0];
         float2 be = cadd(v[1], v[4]), bm = csub(v[1], v[4]);
         float2 cd = cadd(v[2], v[3]), cm = csub(v[2], v[3]);
@@ -123,25 +189,54 @@ __device__ __forceinline__ void dft(float2 *v, int sign, const float2 *roots, in
         v[4] = make_float2(r1.x + i1.y, r1.y - i1.x);
         v[2] = make_float2(r2.x - i2.y, r2.y + i2.x);
         v[3] = make_float2(r2.x + i2.y, r2.y - i2.x);
+    } else {
+        constexpr int A = radix_a(R), B = R / A;
+        static_assert(A > 1 && A * B == R, "unsupported radix");
+        float2 y[R];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            float2 t[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) t[b] = v[a + A * b];
+            dft<B, SIGN>(t);
+#pragma unroll
+            for (int kb = 0; kb < B; ++kb) {
+                if (a * kb == 0) {
+                    y[a * B + kb] = t[kb];
+                } else {
+                    const float wr = (float)tw_re<R, SIGN>(a * kb), wi = (float)tw_im<R, SIGN>(a * kb);
+                    y[a * B + kb] = make_float2(t[kb].x * wr - t[kb].y * wi, t[kb].x * wi + t[kb].y * wr);
+                }
+            }
+        }
+#pragma unroll
+        for (int kb = 0; kb < B; ++kb) {
+            float2 t[A];
+#pragma unroll
+            for (int a = 0; a < A; ++a) t[a] = y[a * B + kb];
+            dft<A, SIGN>(t);
+#pragma unroll
+            for (int ka = 0; ka < A; ++ka) v[kb + B * ka] = t[ka];
+        }
     }
 }
 
-// One radix-R Stockham stage over the tile's lines (SUB = false: L lines of
-// length m) or over a sub-line set (length len).
-template <int R, int INNER, bool SUB>
-__device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P, const SubLines &g, int len, int Ns,
-                                      int sign, const float2 *roots, const FastDiv &fNbf, const FastDiv &fNs) {
-    constexpr int MAXB = (kLdsComplex / R + kThreads - 1) / kThreads;
+// One radix-R Stockham stage over a (sub-)line set of length len.
+template <int R, int INNER, int NT>
+__device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P, const SubLines &g, bool sub, int len,
+                                      int Ns, int sign, const float2 *roots, const FastDiv &fNbf,
+                                      const FastDiv &fNs) {
+    constexpr int MAXB = (kLdsComplex / R + NT - 1) / NT;
     const int nbf = len / R;
-    const int tot = (SUB ? g.nv : P.L) * nbf;
+    const int tot = g.nv * nbf;
     const int tstep = len / (Ns * R);
     float2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int e = threadIdx.x + b * kThreads;
+        const int e = threadIdx.x + b * NT;
         if (e < tot) {
             int j, base, step;
-            locate<INNER, SUB>(P, g, fNbf, nbf, e, j, base, step);
+            locate<INNER>(P, g, sub, fNbf, nbf, e, j, base, step);
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = buf[base + (j + t * nbf) * step];
         }
@@ -149,41 +244,24 @@ __device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int e = threadIdx.x + b * kThreads;
+        const int e = threadIdx.x + b * NT;
         if (e < tot) {
             int j, base, step;
-            locate<INNER, SUB>(P, g, fNbf, nbf, e, j, base, step);
+            locate<INNER>(P, g, sub, fNbf, nbf, e, j, base, step);
             const int jq = (int)fNs.div((unsigned)j);
             const int k = j - jq * Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], twiddle(roots, t * k * tstep, sign));
             }
-            dft<R>(v[b], sign, roots, len / R);
+            if (sign < 0) dft<R, -1>(v[b]);
+            else dft<R, 1>(v[b]);
             const int o = jq * Ns * R + k;
 #pragma unroll
             for (int t = 0; t < R; ++t) buf[base + (o + t * Ns) * step] = v[b][t];
         }
     }
     __syncthreads();
-}
-
-// Stockham stages rad[first, first + count) over (sub-)lines of length len
-// whose root table holds exp(-2 pi i k / len).
-template <int INNER, bool SUB>
-__device__ void stages(float2 *buf, const KParams &P, const SubLines &g, int len, int first, int count, int sign,
-                       const float2 *roots) {
-    int Ns = 1;
-    for (int s = first; s < first + count; ++s) {
-        const int r = P.rad[s];
-        switch (r) {
-            case 2: stage<2, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
-            case 3: stage<3, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
-            case 4: stage<4, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
-            default: stage<5, INNER, SUB>(buf, P, g, len, Ns, sign, roots, P.fNbf[s], P.fNs[s]); break;
-        }
-        Ns *= r;
-    }
 }
 
 // Rader's step between the two convolution FFTs.  Row c of line l holds y0 (the
@@ -197,7 +275,7 @@ __device__ __forceinline__ void rader_pointwise(float2 *buf, const KParams &P) {
     const int q1 = P.Q - 1;
     for (int e = threadIdx.x; e < g.nv * q1; e += blockDim.x) {
         int j, base, step;
-        locate<INNER, true>(P, g, P.fQ1, q1, e, j, base, step);
+        locate<INNER>(P, g, true, P.fQ1, q1, e, j, base, step);
         const int id = base + j * step;
         const float2 A = buf[id];
         float2 c = cmul(A, P.bhat[j]);
@@ -234,8 +312,8 @@ __device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, i
     return true;
 }
 
-template <int INNER>
-__global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
+template <int INNER, int NT>
+__global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
     extern __shared__ float2 buf[];
     const int L = P.L, M = P.m, N = P.n;
     const AxisPass &a = P.a;
@@ -297,25 +375,34 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     }
     __syncthreads();
 
-    const SubLines whole;
-    if (P.kind == KIND_STOCKHAM) {
-        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, P.sign, P.roots);
-    } else if (P.kind == KIND_RADER) {
-        if (P.S > 1) stages<INNER, true>(buf, P, P.sub_s, P.S, 0, P.nrad_s, P.sign, P.roots);
-        stages<INNER, true>(buf, P, P.sub_q, P.Q - 1, P.nrad_s, P.nrad - P.nrad_s, -1, P.roots_q);
-        rader_pointwise<INNER>(buf, P);
-        stages<INNER, true>(buf, P, P.sub_q, P.Q - 1, P.nrad_s, P.nrad - P.nrad_s, +1, P.roots_q);
-    } else {
-        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, -1, P.roots);
-        for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
-            int l, n;
-            if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
-            else { l = (int)P.fM.div((unsigned)e); n = e - l * M; }
-            const int id = lidx<INNER>(l, n, L, M);
-            buf[id] = cmul(buf[id], P.bhat[n]);
+    for (int s = 0; s < P.nst; ++s) {
+        const int gs = P.st_g[s];
+        const SubLines &g = gs == 2 ? P.sub_q : gs == 1 ? P.sub_s : P.whole;
+        const float2 *roots = gs == 2 ? P.roots_q : P.roots;
+#define LAM_STAGE(R)                                                                                              \
+    case R:                                                                                                       \
+        stage<R, INNER, NT>(buf, P, g, gs != 0, P.st_len[s], P.st_ns[s], P.st_sign[s], roots, P.fNbf[s], P.fNs[s]); \
+        break;
+        switch (P.rad[s]) {
+            LAM_STAGE(2) LAM_STAGE(3) LAM_STAGE(4) LAM_STAGE(5) LAM_STAGE(6) LAM_STAGE(8) LAM_STAGE(9)
+            LAM_STAGE(10) LAM_STAGE(12)
+            default: break;
         }
-        __syncthreads();
-        stages<INNER, false>(buf, P, whole, M, 0, P.nrad, +1, P.roots);
+#undef LAM_STAGE
+        if (s == P.pw_after) {
+            if (P.kind == KIND_RADER) {
+                rader_pointwise<INNER>(buf, P);
+            } else {
+                for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
+                    int l, n;
+                    if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+                    else { l = (int)P.fM.div((unsigned)e); n = e - l * M; }
+                    const int id = lidx<INNER>(l, n, L, M);
+                    buf[id] = cmul(buf[id], P.bhat[n]);
+                }
+                __syncthreads();
+            }
+        }
     }
 
     for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
@@ -342,19 +429,44 @@ __global__ __launch_bounds__(kThreads) void k_fft_axis(KParams P) {
     }
 }
 
-// radix list for a length (fours first, then 2, 3, 5); false if another prime
-// factor remains (such lengths go through Bluestein)
+// Radices with a register kernel (dft): the base radices 2..5 and composites
+// up to 12.  Fewer, larger stages mean fewer LDS round trips, barriers and
+// index splits per element (90 = 10 * 9 in two stages instead of four).
+// Measured: radices up to 16 or 20 raise the register count (more waves per
+// workgroup stall at the barriers) and gain nothing over 12.
+constexpr int kRadices[] = {12, 10, 9, 8, 6, 5, 4, 3, 2};
+
+// fewest stages; among those the largest smallest radix (no trailing radix-2 stage
+// where a balanced split exists); radices in non-increasing order
+void split_search(int n, int maxr, std::vector<int> &cur, std::vector<int> &best) {
+    if (n == 1) {
+        const auto smallest = [](const std::vector<int> &v) { return v.empty() ? 0 : v.back(); };
+        if (best.empty() || cur.size() < best.size() ||
+            (cur.size() == best.size() && smallest(cur) > smallest(best)))
+            best = cur;
+        return;
+    }
+    if (!best.empty() && cur.size() + 1 > best.size()) return;
+    for (int r : kRadices) {
+        if (r > maxr || n % r || (!cur.empty() && r > cur.back())) continue;
+        cur.push_back(r);
+        split_search(n / r, maxr, cur, best);
+        cur.pop_back();
+    }
+}
+
+// radix list for a 2,3,5-smooth length; false if another prime factor remains
 bool factor_len(int n, int *rad, int &nrad) {
     nrad = 0;
     int r = n;
-    int twos = 0;
-    while (r % 2 == 0) { r /= 2; ++twos; }
-    for (int k = 0; k < twos / 2; ++k) rad[nrad++] = 4;
-    if (twos % 2) rad[nrad++] = 2;
-    for (int p : {3, 5}) {
-        while (r % p == 0) { rad[nrad++] = p; r /= p; }
-    }
-    return r == 1;
+    for (int p : {2, 3, 5}) while (r % p == 0) r /= p;
+    if (r != 1) return false;
+    if (n == 1) return true;
+    std::vector<int> cur, best;
+    split_search(n, 12, cur, best);
+    if (best.empty() || best.size() > 24) return false;
+    for (int x : best) rad[nrad++] = x;
+    return true;
 }
 
 bool smooth235(int n) {
@@ -566,8 +678,6 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.n = plan.n;
     P.m = plan.m;
     P.kind = plan.rader ? KIND_RADER : plan.blue ? KIND_BLUESTEIN : KIND_STOCKHAM;
-    P.nrad = plan.nrad;
-    for (int i = 0; i < 24; ++i) P.rad[i] = plan.rad[i];
     P.sign = plan.sign;
     P.roots = plan.roots.as<float2>();
     P.chirp = plan.chirp.as<float2>();
@@ -576,7 +686,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     const int cap = std::max(1, std::min(kLdsComplex / plan.m, kMaxLines));
     dim3 grid;
     if (inner) {
-        P.Li = std::min({a.ci, cap, 16});
+        P.Li = std::min({a.ci, cap, kInnerLines});
         P.Lo = std::max(1, std::min(a.c2, cap / P.Li));
         grid = dim3((unsigned)((a.ci + P.Li - 1) / P.Li), (unsigned)((a.c2 + P.Lo - 1) / P.Lo), (unsigned)a.c1);
     } else {
@@ -594,19 +704,32 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.fM = FastDiv((unsigned)plan.m);
     P.fLi = FastDiv((unsigned)P.Li);
     P.fPad = FastDiv((unsigned)std::max(1, plan.m - plan.n));
-    if (plan.rader) {
-        // stages over S, then over Q - 1, each with its own spans
+    // the flattened stage list: runs of the plan's radices over a (sub-)line set
+    P.nst = 0;
+    P.pw_after = -1;
+    auto add_run = [&](int first, int count, int len, int sign, int g) {
         int Ns = 1;
-        for (int s = 0; s < plan.nrad; ++s) {
-            if (s == plan.nrad_s) Ns = 1;
-            const int len = s < plan.nrad_s ? plan.S : plan.Q - 1;
-            P.fNbf[s] = FastDiv((unsigned)(len / plan.rad[s]));
-            P.fNs[s] = FastDiv((unsigned)Ns);
+        for (int s = first; s < first + count; ++s) {
+            LAM_REQUIRE(P.nst < 24, LAM_ERR_ARG, "FFT plan has too many stages");
+            const int i = P.nst++;
+            P.rad[i] = plan.rad[s];
+            P.st_len[i] = len;
+            P.st_sign[i] = sign;
+            P.st_g[i] = g;
+            P.st_ns[i] = Ns;
+            P.fNbf[i] = FastDiv((unsigned)(len / plan.rad[s]));
+            P.fNs[i] = FastDiv((unsigned)Ns);
             Ns *= plan.rad[s];
         }
+    };
+    P.whole.nv = P.L; P.whole.C = 1; P.whole.off = 0; P.whole.coff = 0; P.whole.str = 1;
+    P.whole.fnv = FastDiv((unsigned)P.L); P.whole.fC = FastDiv(1u);
+    P.S = P.Q = 1;
+    P.roots_q = nullptr;
+    P.pos_in = P.pos_out = nullptr;
+    if (plan.rader) {
         P.S = plan.S;
         P.Q = plan.Q;
-        P.nrad_s = plan.nrad_s;
         P.roots_q = plan.roots_q.as<float2>();
         P.pos_in = plan.pos.as<short>();
         P.pos_out = P.pos_in + plan.n;
@@ -617,25 +740,25 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         // Q - 1 point convolutions: S rows per line, element t of row c at c * Q + 1 + t
         P.sub_q.nv = P.L * plan.S; P.sub_q.C = plan.S; P.sub_q.off = 1; P.sub_q.coff = plan.Q; P.sub_q.str = 1;
         P.sub_q.fnv = FastDiv((unsigned)P.sub_q.nv); P.sub_q.fC = FastDiv((unsigned)plan.S);
+        const int nq = plan.nrad - plan.nrad_s;
+        add_run(0, plan.nrad_s, plan.S, plan.sign, 1);
+        add_run(plan.nrad_s, nq, plan.Q - 1, -1, 2);
+        P.pw_after = P.nst - 1;
+        add_run(plan.nrad_s, nq, plan.Q - 1, +1, 2);
+    } else if (plan.blue) {
+        add_run(0, plan.nrad, plan.m, -1, 0);
+        P.pw_after = P.nst - 1;
+        add_run(0, plan.nrad, plan.m, +1, 0);
     } else {
-        int Ns = 1;
-        for (int s = 0; s < plan.nrad; ++s) {
-            P.fNbf[s] = FastDiv((unsigned)(plan.m / plan.rad[s]));
-            P.fNs[s] = FastDiv((unsigned)Ns);
-            Ns *= plan.rad[s];
-        }
-        P.S = P.Q = 1;
-        P.nrad_s = 0;
-        P.roots_q = nullptr;
-        P.pos_in = P.pos_out = nullptr;
+        add_run(0, plan.nrad, plan.m, plan.sign, 0);
     }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L;
     TimedScope ts(cx, "fft_axis");
     if (inner)
-        hipLaunchKernelGGL(k_fft_axis<1>, grid, dim3(kThreads), lds, cx.stream, P);
+        hipLaunchKernelGGL((k_fft_axis<1, kThreads>), grid, dim3(kThreads), lds, cx.stream, P);
     else
-        hipLaunchKernelGGL(k_fft_axis<0>, grid, dim3(kThreads), lds, cx.stream, P);
+        hipLaunchKernelGGL((k_fft_axis<0, kThreads>), grid, dim3(kThreads), lds, cx.stream, P);
     check_launch();
 }
 
