@@ -175,6 +175,11 @@ lam_status lam_autotune_bins(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3]
  * Enabling clears earlier records.  (No reference counterpart; the reference
  * only has a wall-clock Timer, include/timer.h:31-64.) */
 lam_status lam_timing_enable(lam_ctx *ctx, int on);
+/* Restrict the timing to the scopes of one kernel name (NULL or "": all).
+ * Each timed scope adds two event records to the stream (about 5 us per
+ * scope on MI355X), so a throughput measurement times only the kernel it
+ * reports on. */
+lam_status lam_timing_only(lam_ctx *ctx, const char *name);
 /* Total device milliseconds and launch count recorded under a kernel name:
  * "interp", "spread", "bin_points", "fft_axis", "deapod_pad", "crop_deapod",
  * "toep_build", "toep_mul". */

@@ -102,6 +102,7 @@ _SIGS = {
     "lam_toep_op": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     "lam_autotune_bins": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int), c_dp]),
     "lam_timing_enable": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "lam_timing_only": (ctypes.c_int, [c_vp, ctypes.c_char_p]),
     "lam_timing_query": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_dp, c_i64p]),
     "lam_mbir": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
                                 ctypes.POINTER(lam_recon_params), c_vp, ctypes.POINTER(lam_recon_stats)]),

@@ -698,6 +698,15 @@ lam_status lam_timing_enable(lam_ctx *ctx, int on) {
     });
 }
 
+lam_status lam_timing_only(lam_ctx *ctx, const char *name) {
+    return guard([&] {
+        need(ctx, "ctx");
+        Ctx &c = ctx->impl;
+        LAM_HIP(hipStreamSynchronize(c.stream));
+        c.timer.only = name ? name : "";
+    });
+}
+
 lam_status lam_timing_query(lam_ctx *ctx, const char *name, double *total_ms, int64_t *count) {
     return guard([&] {
         need(ctx, "ctx"); need(name, "name");

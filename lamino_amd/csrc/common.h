@@ -134,6 +134,7 @@ struct Ctx;
 // HIP-event timing of named kernels on the context stream (off by default).
 struct KernelTimer {
     bool on = false;
+    std::string only;   // when non-empty: record only scopes of this name
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> ev;
     std::vector<hipEvent_t> pool;
     hipEvent_t take() {
@@ -188,7 +189,7 @@ struct TimedScope {
     const char *name;
     hipEvent_t stop = nullptr;
     TimedScope(Ctx &c, const char *n) : cx(c), name(n) {
-        if (!cx.timer.on) return;
+        if (!cx.timer.on || (!cx.timer.only.empty() && cx.timer.only != name)) return;
         hipEvent_t start = cx.timer.take();
         stop = cx.timer.take();
         LAM_HIP(hipEventRecord(start, cx.stream));
