@@ -62,6 +62,7 @@ struct RowPlan {
     int ylo = 0, ly = 0;               // the y planes the rows touch: [ylo, ylo + ly) mod nfy (ly 0: all)
     DevBuf wy;                         // float [Jn][W] row weights
     DevBuf iy;                         // int [Jn] unwrapped first y cell of each row
+    std::vector<int> iy_host;          // the same on the host
     DevBuf count;                      // unsigned [Jn][ntiles] points per (row, tile)
     DevBuf ids;                        // half-point ids sorted by (row, tile, first cell)
     DevBuf chunks;                     // uint4 (row * ntiles + tile, first, count, 0), one per non-empty (row, tile)
