@@ -39,12 +39,14 @@ struct FftPlan {
     int S = 1, Q = 1, nrad_s = 0;
     int nrad = 0;
     int rad[24] = {};
-    DevBuf roots;   // m entries exp(-2 pi i k / m)   (Rader: S entries)
     DevBuf chirp;   // n entries exp(sign pi i k^2 / n)        (Bluestein)
     DevBuf bhat;    // m entries FFT_m(conj chirp, wrapped) / m (Bluestein);
                     // Rader: Q - 1 entries FFT_{Q-1}(b) / (Q - 1), b_j = exp(sign 2 pi i g^j / Q)
-    DevBuf roots_q; // Rader: Q - 1 entries exp(-2 pi i k / (Q - 1))
     DevBuf pos;     // Rader: 2n shorts, the LDS slot of input element k, then of output element k
+    // per stage of the transform's stage list: twiddles exp(sign 2 pi i t k / (Ns R)),
+    // t = 1..R-1, for k < Ns, stored [k][t - 1] from float2 offset tw_off[stage]
+    DevBuf tw;
+    int tw_off[24] = {};
 };
 
 struct AxisPass {

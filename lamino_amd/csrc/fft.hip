@@ -41,22 +41,22 @@ enum { KIND_STOCKHAM = 0, KIND_BLUESTEIN = 1, KIND_RADER = 2 };
 //
 // The transform is a list of Stockham stages, each over a (sub-)line set:
 // st_g = 0: the tile's L lines of length m; 1: the S-point columns and 2: the
-// (Q - 1)-point rows of a Rader plan (roots_q).  Bluestein and Rader plans apply
+// (Q - 1)-point rows of a Rader plan.  Bluestein and Rader plans apply
 // their pointwise product after stage pw_after.
 struct KParams {
     AxisPass a;
     int n, m, kind, sign;
     int L, Li, Lo;  // lines per workgroup, L = Li * Lo
     int nst, pw_after;
-    int rad[24], st_len[24], st_sign[24], st_g[24], st_ns[24];
-    const float2 *roots, *chirp, *bhat;
+    int rad[24], st_len[24], st_sign[24], st_g[24], st_ns[24], st_tw[24];
+    const float2 *tw;                     // the plan's twiddle tables (stage s from st_tw[s])
+    const float2 *chirp, *bhat;
     FastDiv fL, fN, fM, fLi, fPad;        // tile lines, length, inner length, lines per o2 row, M - N
     FastDiv fNbf[24], fNs[24];            // per stage: butterflies per line (len / R), span Ns
     int isn, osn;                         // element strides along the transform (fit 32 bits)
     // Rader plans: n = S * Q; the S-point transforms run over the Q columns of
     // each line (sub_s), the Q - 1 point convolutions over its S rows (sub_q)
     int S, Q;
-    const float2 *roots_q;
     const short *pos_in, *pos_out;
     SubLines whole, sub_s, sub_q;
     FastDiv fQ1;                          // Q - 1
@@ -90,12 +90,6 @@ __device__ __forceinline__ void locate(const KParams &P, const SubLines &g, bool
     const int p0 = g.off + c * g.coff;
     base = INNER ? p0 * P.L + l : l * P.m + p0;
     step = INNER ? g.str * P.L : g.str;
-}
-
-__device__ __forceinline__ float2 twiddle(const float2 *roots, int idx, int sign) {
-    float2 w = roots[idx];
-    if (sign > 0) w.y = -w.y;
-    return w;
 }
 
 // Compile-time cos/sin (Taylor series on [-pi, pi], double) for the internal
@@ -224,12 +218,11 @@ __device__ __forceinline__ void dft(float2 *v) {
 // One radix-R Stockham stage over a (sub-)line set of length len.
 template <int R, int INNER, int NT>
 __device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P, const SubLines &g, bool sub, int len,
-                                      int Ns, int sign, const float2 *roots, const FastDiv &fNbf,
+                                      int Ns, int sign, const float2 *__restrict__ tw, const FastDiv &fNbf,
                                       const FastDiv &fNs) {
     constexpr int MAXB = (kLdsComplex / R + NT - 1) / NT;
     const int nbf = len / R;
     const int tot = g.nv * nbf;
-    const int tstep = len / (Ns * R);
     float2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
@@ -251,8 +244,9 @@ __device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P
             const int jq = (int)fNs.div((unsigned)j);
             const int k = j - jq * Ns;
             if (Ns > 1) {
+                const float2 *w = tw + k * (R - 1);
 #pragma unroll
-                for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], twiddle(roots, t * k * tstep, sign));
+                for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], w[t - 1]);
             }
             if (sign < 0) dft<R, -1>(v[b]);
             else dft<R, 1>(v[b]);
@@ -378,10 +372,10 @@ __global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
     for (int s = 0; s < P.nst; ++s) {
         const int gs = P.st_g[s];
         const SubLines &g = gs == 2 ? P.sub_q : gs == 1 ? P.sub_s : P.whole;
-        const float2 *roots = gs == 2 ? P.roots_q : P.roots;
-#define LAM_STAGE(R)                                                                                              \
-    case R:                                                                                                       \
-        stage<R, INNER, NT>(buf, P, g, gs != 0, P.st_len[s], P.st_ns[s], P.st_sign[s], roots, P.fNbf[s], P.fNs[s]); \
+        const float2 *tw = P.tw + P.st_tw[s];
+#define LAM_STAGE(R)                                                                                           \
+    case R:                                                                                                    \
+        stage<R, INNER, NT>(buf, P, g, gs != 0, P.st_len[s], P.st_ns[s], P.st_sign[s], tw, P.fNbf[s], P.fNs[s]); \
         break;
         switch (P.rad[s]) {
             LAM_STAGE(2) LAM_STAGE(3) LAM_STAGE(4) LAM_STAGE(5) LAM_STAGE(6) LAM_STAGE(8) LAM_STAGE(9)
@@ -512,16 +506,6 @@ int inverse_mod(int a, int m) {  // m > 1, gcd(a, m) = 1
     return 0;
 }
 
-std::vector<float2> root_table(int m) {
-    const long double twopi = 6.283185307179586476925286766559L;
-    std::vector<float2> r(m);
-    for (int k = 0; k < m; ++k) {
-        long double ang = -twopi * (long double)k / (long double)m;
-        r[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
-    return r;
-}
-
 // Prime-factor (Good-Thomas) + Rader tables.  Input element k goes to row
 // n1 = k u1 mod S, column n2 = k u2 mod Q (u1 = Q^-1 mod S, u2 = S^-1 mod Q);
 // output element k comes from row k mod S, column k mod Q; then
@@ -566,15 +550,70 @@ void build_rader(Ctx &cx, FftPlan &plan) {
         acc /= (double)q1;
         bh[f] = make_float2((float)acc.real(), (float)acc.imag());
     }
-    const auto rs = root_table(S), rq = root_table(q1);
-    plan.roots.alloc(sizeof(float2) * S);
-    upload_now(cx, plan.roots.ptr, rs.data(), sizeof(float2) * S);
-    plan.roots_q.alloc(sizeof(float2) * q1);
-    upload_now(cx, plan.roots_q.ptr, rq.data(), sizeof(float2) * q1);
     plan.bhat.alloc(sizeof(float2) * q1);
     upload_now(cx, plan.bhat.ptr, bh.data(), sizeof(float2) * q1);
     plan.pos.alloc(sizeof(short) * pos.size());
     upload_now(cx, plan.pos.ptr, pos.data(), sizeof(short) * pos.size());
+}
+
+struct StageDesc {
+    int rad, len, sign, g, ns;
+};
+
+// The plan's flattened stage list: runs of its radices over a (sub-)line set
+// (g: 0 the whole lines, 1 the S-point columns, 2 the (Q - 1)-point rows of a
+// Rader plan); pw_after is the stage after which Bluestein's or Rader's
+// pointwise product runs (-1: none).
+std::vector<StageDesc> stage_list(const FftPlan &plan, int &pw_after) {
+    std::vector<StageDesc> st;
+    pw_after = -1;
+    auto run = [&](int first, int count, int len, int sign, int g) {
+        int Ns = 1;
+        for (int s = first; s < first + count; ++s) {
+            st.push_back({plan.rad[s], len, sign, g, Ns});
+            Ns *= plan.rad[s];
+        }
+    };
+    if (plan.rader) {
+        const int nq = plan.nrad - plan.nrad_s;
+        run(0, plan.nrad_s, plan.S, plan.sign, 1);
+        run(plan.nrad_s, nq, plan.Q - 1, -1, 2);
+        pw_after = (int)st.size() - 1;
+        run(plan.nrad_s, nq, plan.Q - 1, +1, 2);
+    } else if (plan.blue) {
+        run(0, plan.nrad, plan.m, -1, 0);
+        pw_after = (int)st.size() - 1;
+        run(0, plan.nrad, plan.m, +1, 0);
+    } else {
+        run(0, plan.nrad, plan.m, plan.sign, 0);
+    }
+    LAM_REQUIRE(st.size() <= 24, LAM_ERR_ARG, "FFT plan has too many stages");
+    return st;
+}
+
+// Twiddles of every stage, exp(sign 2 pi i t k / (Ns R)) for k < Ns and
+// t = 1..R-1, laid out [k][t - 1] so that a butterfly reads its R - 1 values
+// from one base address (computed in long double, rounded once).
+void build_twiddles(Ctx &cx, FftPlan &plan) {
+    int pw;
+    const auto st = stage_list(plan, pw);
+    const long double twopi = 6.283185307179586476925286766559L;
+    std::vector<float2> tw;
+    for (size_t i = 0; i < st.size(); ++i) {
+        plan.tw_off[i] = (int)tw.size();
+        const StageDesc &d = st[i];
+        if (d.ns <= 1) continue;
+        const long long span = (long long)d.ns * d.rad;
+        for (int k = 0; k < d.ns; ++k)
+            for (int t = 1; t < d.rad; ++t) {
+                const long double ang = (long double)d.sign * twopi * (long double)((long long)t * k % span) /
+                                        (long double)span;
+                tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+            }
+    }
+    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    plan.tw.alloc(sizeof(float2) * tw.size());
+    upload_now(cx, plan.tw.ptr, tw.data(), sizeof(float2) * tw.size());
 }
 
 }  // namespace
@@ -609,6 +648,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         for (int i = 0; i < ns; ++i) plan->rad[i] = rs[i];
         for (int i = 0; i < nq; ++i) plan->rad[ns + i] = rq[i];
         build_rader(cx, *plan);
+        build_twiddles(cx, *plan);
         FftPlan &ref = *plan;
         cx.fft_plans.emplace(key, std::move(plan));
         return ref;
@@ -624,13 +664,6 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
     const int m = plan->m;
     LAM_REQUIRE(m <= kLdsComplex, LAM_ERR_ARG, "FFT length " + std::to_string(n) + " exceeds the LDS transform limit");
     const long double twopi = 6.283185307179586476925286766559L;
-    std::vector<float2> roots(m);
-    for (int k = 0; k < m; ++k) {
-        long double ang = -twopi * (long double)k / (long double)m;
-        roots[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
-    plan->roots.alloc(sizeof(float2) * m);
-    upload_now(cx, plan->roots.ptr, roots.data(), sizeof(float2) * m);
     if (plan->blue) {
         std::vector<std::complex<double>> ch(n), b(m, 0.0), bh(m);
         std::vector<float2> chf(n), bhf(m);
@@ -666,6 +699,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         upload_now(cx, plan->chirp.ptr, chf.data(), sizeof(float2) * n);
         upload_now(cx, plan->bhat.ptr, bhf.data(), sizeof(float2) * m);
     }
+    build_twiddles(cx, *plan);
     FftPlan &ref = *plan;
     cx.fft_plans.emplace(key, std::move(plan));
     return ref;
@@ -679,7 +713,6 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.m = plan.m;
     P.kind = plan.rader ? KIND_RADER : plan.blue ? KIND_BLUESTEIN : KIND_STOCKHAM;
     P.sign = plan.sign;
-    P.roots = plan.roots.as<float2>();
     P.chirp = plan.chirp.as<float2>();
     P.bhat = plan.bhat.as<float2>();
     const bool inner = a.ci > 1;
@@ -704,33 +737,29 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     P.fM = FastDiv((unsigned)plan.m);
     P.fLi = FastDiv((unsigned)P.Li);
     P.fPad = FastDiv((unsigned)std::max(1, plan.m - plan.n));
-    // the flattened stage list: runs of the plan's radices over a (sub-)line set
-    P.nst = 0;
-    P.pw_after = -1;
-    auto add_run = [&](int first, int count, int len, int sign, int g) {
-        int Ns = 1;
-        for (int s = first; s < first + count; ++s) {
-            LAM_REQUIRE(P.nst < 24, LAM_ERR_ARG, "FFT plan has too many stages");
-            const int i = P.nst++;
-            P.rad[i] = plan.rad[s];
-            P.st_len[i] = len;
-            P.st_sign[i] = sign;
-            P.st_g[i] = g;
-            P.st_ns[i] = Ns;
-            P.fNbf[i] = FastDiv((unsigned)(len / plan.rad[s]));
-            P.fNs[i] = FastDiv((unsigned)Ns);
-            Ns *= plan.rad[s];
-        }
-    };
+    // the plan's flattened stage list with its twiddle tables
+    int pw_after;
+    const auto st = stage_list(plan, pw_after);
+    P.nst = (int)st.size();
+    P.pw_after = pw_after;
+    P.tw = plan.tw.as<float2>();
+    for (int i = 0; i < P.nst; ++i) {
+        P.rad[i] = st[i].rad;
+        P.st_len[i] = st[i].len;
+        P.st_sign[i] = st[i].sign;
+        P.st_g[i] = st[i].g;
+        P.st_ns[i] = st[i].ns;
+        P.st_tw[i] = plan.tw_off[i];
+        P.fNbf[i] = FastDiv((unsigned)(st[i].len / st[i].rad));
+        P.fNs[i] = FastDiv((unsigned)st[i].ns);
+    }
     P.whole.nv = P.L; P.whole.C = 1; P.whole.off = 0; P.whole.coff = 0; P.whole.str = 1;
     P.whole.fnv = FastDiv((unsigned)P.L); P.whole.fC = FastDiv(1u);
     P.S = P.Q = 1;
-    P.roots_q = nullptr;
     P.pos_in = P.pos_out = nullptr;
     if (plan.rader) {
         P.S = plan.S;
         P.Q = plan.Q;
-        P.roots_q = plan.roots_q.as<float2>();
         P.pos_in = plan.pos.as<short>();
         P.pos_out = P.pos_in + plan.n;
         P.fQ1 = FastDiv((unsigned)(plan.Q - 1));
@@ -740,17 +769,6 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         // Q - 1 point convolutions: S rows per line, element t of row c at c * Q + 1 + t
         P.sub_q.nv = P.L * plan.S; P.sub_q.C = plan.S; P.sub_q.off = 1; P.sub_q.coff = plan.Q; P.sub_q.str = 1;
         P.sub_q.fnv = FastDiv((unsigned)P.sub_q.nv); P.sub_q.fC = FastDiv((unsigned)plan.S);
-        const int nq = plan.nrad - plan.nrad_s;
-        add_run(0, plan.nrad_s, plan.S, plan.sign, 1);
-        add_run(plan.nrad_s, nq, plan.Q - 1, -1, 2);
-        P.pw_after = P.nst - 1;
-        add_run(plan.nrad_s, nq, plan.Q - 1, +1, 2);
-    } else if (plan.blue) {
-        add_run(0, plan.nrad, plan.m, -1, 0);
-        P.pw_after = P.nst - 1;
-        add_run(0, plan.nrad, plan.m, +1, 0);
-    } else {
-        add_run(0, plan.nrad, plan.m, plan.sign, 0);
     }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L;
