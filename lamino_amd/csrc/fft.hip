@@ -21,6 +21,10 @@ constexpr int kThreads = 256;
 constexpr int kLdsComplex = 3840;
 constexpr int kMaxLines = 128;
 
+// Elements per thread whose global loads (load phase) or LDS reads (store
+// phase) are issued together.
+constexpr int kLoadBatch = 8;
+
 // Lines per tile along the inner (contiguous) index, LDS permitting: measured
 // 16 -> 48 took the 90-point passes of the benchmark grid from 94 to 76 us.
 constexpr int kInnerLines = 48;
@@ -319,9 +323,11 @@ __global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
     const int o1w = remap((int)blockIdx.z, a.o1_split, a.o1_gap);
     const bool has2 = o1w < a.c1_in2, put2 = o1w < a.c1_out2;
 
-    // 64-bit offsets of the tile's lines, once per workgroup (-1: line past the set)
+    // 64-bit offsets of the tile's lines, once per workgroup (-1: line past the set),
+    // and a Rader plan's slot tables (input, then output)
     long long *base_in = reinterpret_cast<long long *>(buf + L * M);
     long long *base_out = base_in + L;
+    short *pos_lds = reinterpret_cast<short *>(base_out + L);
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
         long long o1;
         int o2, i;
@@ -329,34 +335,80 @@ __global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
         base_in[l] = ok ? o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi : -1;
         base_out[l] = ok ? o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi : -1;
     }
+    if (P.kind == KIND_RADER)
+        for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) pos_lds[i] = P.pos_in[i];
     __syncthreads();
 
-    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
-        int l, n;
-        if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
-        else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
-        int mm = n + a.rot_in;
-        if (mm >= N) mm -= N;
-        float2 v = make_float2(0.f, 0.f);
-        const long long lb = base_in[l];
-        if (lb >= 0 && (mm < nv || mm >= N - a.nv2) && in_window(mm, a.ld_lo, a.ld_len, N)) {
-            const long long off = lb + (long long)mm * P.isn;
-            if (a.ld_mode == LD_COMPLEX) v = static_cast<const float2 *>(a.in)[off];
-            else if (a.ld_mode == LD_REAL) v.x = static_cast<const float *>(a.in)[off];
-            else if (a.ld_mode == LD_REAL_PAIR) {
-                v.x = static_cast<const float *>(a.in)[off];
-                if (has2) v.y = static_cast<const float *>(a.in2)[off];
-            } else {
-                v = static_cast<const float2 *>(a.in)[off];
-                if (has2) {
-                    const float2 b = static_cast<const float2 *>(a.in2)[off];
-                    v.x -= b.y;
-                    v.y += b.x;
-                }
+    // Load: each thread computes the addresses of kLoadBatch elements, issues
+    // their global loads, and only then writes them to LDS, so that that many
+    // loads per thread are in flight (one at a time, the passes ran at a
+    // latency-bound ~2 TB/s).
+    const int LN = L * N;
+    for (int e0 = threadIdx.x; e0 < LN; e0 += kLoadBatch * NT) {
+        long long off[kLoadBatch];
+        int line[kLoadBatch], nidx[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * NT;
+            off[u] = -1;
+            line[u] = -1;
+            nidx[u] = 0;
+            if (e < LN) {
+                int l, n;
+                if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+                else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
+                int mm = n + a.rot_in;
+                if (mm >= N) mm -= N;
+                const long long lb = base_in[l];
+                if (lb >= 0 && (mm < nv || mm >= N - a.nv2) && in_window(mm, a.ld_lo, a.ld_len, N))
+                    off[u] = lb + (long long)mm * P.isn;
+                line[u] = l;
+                nidx[u] = n;
             }
         }
-        if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[n]);
-        buf[lidx<INNER>(l, P.kind == KIND_RADER ? (int)P.pos_in[n] : n, L, M)] = v;
+        float2 val[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) val[u] = make_float2(0.f, 0.f);
+        if (a.ld_mode == LD_COMPLEX) {
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (off[u] >= 0) val[u] = static_cast<const float2 *>(a.in)[off[u]];
+        } else if (a.ld_mode == LD_REAL) {
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (off[u] >= 0) val[u].x = static_cast<const float *>(a.in)[off[u]];
+        } else if (a.ld_mode == LD_REAL_PAIR) {
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (off[u] >= 0) {
+                    val[u].x = static_cast<const float *>(a.in)[off[u]];
+                    if (has2) val[u].y = static_cast<const float *>(a.in2)[off[u]];
+                }
+        } else {
+            float2 sec[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) {
+                sec[u] = make_float2(0.f, 0.f);
+                if (off[u] >= 0) {
+                    val[u] = static_cast<const float2 *>(a.in)[off[u]];
+                    if (has2) sec[u] = static_cast<const float2 *>(a.in2)[off[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) {
+                val[u].x -= sec[u].y;
+                val[u].y += sec[u].x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            if (line[u] >= 0) {
+                float2 v = val[u];
+                if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
+                const int p = P.kind == KIND_RADER ? (int)pos_lds[nidx[u]] : nidx[u];
+                buf[lidx<INNER>(line[u], p, L, M)] = v;
+            }
+        }
     }
     if (P.kind == KIND_BLUESTEIN) {
         const int pad = M - N;
@@ -399,25 +451,49 @@ __global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
         }
     }
 
-    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
-        int l, n;
-        if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
-        else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
-        int mm = n + a.rot_out;
-        if (mm >= N) mm -= N;
-        const long long lb = base_out[l];
-        if (lb >= 0 && (mm < ns || mm >= N - a.ns2) && in_window(mm, a.st_lo, a.st_len, N)) {
-            float2 v = buf[lidx<INNER>(l, P.kind == KIND_RADER ? (int)P.pos_out[n] : n, L, M)];
-            if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[n]);
+    // Store, kLoadBatch elements per thread at a time: their LDS reads are issued
+    // together, then their global stores.
+    for (int e0 = threadIdx.x; e0 < LN; e0 += kLoadBatch * NT) {
+        long long off[kLoadBatch];
+        int line[kLoadBatch], nidx[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = e0 + u * NT;
+            off[u] = -1;
+            line[u] = 0;
+            nidx[u] = 0;
+            if (e < LN) {
+                int l, n;
+                if (INNER) { n = (int)P.fL.div((unsigned)e); l = e - n * L; }
+                else { l = (int)P.fN.div((unsigned)e); n = e - l * N; }
+                int mm = n + a.rot_out;
+                if (mm >= N) mm -= N;
+                const long long lb = base_out[l];
+                if (lb >= 0 && (mm < ns || mm >= N - a.ns2) && in_window(mm, a.st_lo, a.st_len, N))
+                    off[u] = lb + (long long)mm * P.osn;
+                line[u] = l;
+                nidx[u] = n;
+            }
+        }
+        float2 val[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int p = P.kind == KIND_RADER ? (int)pos_lds[N + nidx[u]] : nidx[u];
+            val[u] = off[u] >= 0 ? buf[lidx<INNER>(line[u], p, L, M)] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            if (off[u] < 0) continue;
+            float2 v = val[u];
+            if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
             v.x *= a.scale;
             v.y *= a.scale;
-            const long long off = lb + (long long)mm * P.osn;
-            if (a.st_mode == ST_COMPLEX) static_cast<float2 *>(a.out)[off] = v;
-            else if (a.st_mode == ST_REAL) static_cast<float *>(a.out)[off] = v.x;
-            else if (a.st_mode == ST_IMAG) static_cast<float *>(a.out)[off] = v.y;
+            if (a.st_mode == ST_COMPLEX) static_cast<float2 *>(a.out)[off[u]] = v;
+            else if (a.st_mode == ST_REAL) static_cast<float *>(a.out)[off[u]] = v.x;
+            else if (a.st_mode == ST_IMAG) static_cast<float *>(a.out)[off[u]] = v.y;
             else {
-                static_cast<float *>(a.out)[off] = v.x;
-                if (put2) static_cast<float *>(a.out2)[off] = v.y;
+                static_cast<float *>(a.out)[off[u]] = v.x;
+                if (put2) static_cast<float *>(a.out2)[off[u]] = v.y;
             }
         }
     }
@@ -771,7 +847,8 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         P.sub_q.fnv = FastDiv((unsigned)P.sub_q.nv); P.sub_q.fC = FastDiv((unsigned)plan.S);
     }
     LAM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, LAM_ERR_ARG, "FFT pass grid exceeds launch limits");
-    const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L;
+    const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L +
+                       (plan.rader ? 2 * sizeof(short) * (size_t)plan.n : 0);
     TimedScope ts(cx, "fft_axis");
     if (inner)
         hipLaunchKernelGGL((k_fft_axis<1, kThreads>), grid, dim3(kThreads), lds, cx.stream, P);
