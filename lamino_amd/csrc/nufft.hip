@@ -684,12 +684,15 @@ __global__ __launch_bounds__(256) void k_rows_contract(const float2 *__restrict_
 // tiles above, left and above-left whose boxes reach (z, x); tiles without points
 // stored nothing and count as zero.  One thread per (z, x, component) owns that
 // fine-grid column (no atomics), with a window of W accumulators in registers.
+// The thread writes each plane of its y window [iy_first, iy_last + W) once;
+// accumulate = 0 stores instead of adding (one row group whose window does not
+// wrap: the planes need no zeroing and the flushes wait on no load).
 template <int W>
 __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, const float2 *__restrict__ slab,
                                                      const float2 *__restrict__ halo, const unsigned *__restrict__ count,
                                                      const float *__restrict__ wy, const int *__restrict__ iy,
                                                      int nfz, int nfy, int nfx, RowGeom rg, int j0, int nrows,
-                                                     int dir) {
+                                                     int dir, int accumulate) {
     const long long nxc = (long long)nfx * 3, total = (long long)nfz * nxc;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
@@ -715,7 +718,7 @@ __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, 
             const int iyr = iy[j];
             while (ybase < iyr) {
                 float2 *dst = col + (long long)wrap_any(ybase, nfy) * nxc;
-                *dst = cadd(*dst, acc[0]);
+                *dst = accumulate ? cadd(*dst, acc[0]) : acc[0];
 #pragma unroll
                 for (int b = 0; b < W - 1; ++b) acc[b] = acc[b + 1];
                 acc[W - 1] = make_float2(0.f, 0.f);
@@ -745,7 +748,7 @@ __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, 
 #pragma unroll
         for (int b = 0; b < W; ++b) {
             float2 *dst = col + (long long)wrap_any(ybase + b, nfy) * nxc;
-            *dst = cadd(*dst, acc[b]);
+            *dst = accumulate ? cadd(*dst, acc[b]) : acc[b];
         }
     }
 }
@@ -1555,7 +1558,7 @@ void interp_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, RowPlan &rp) {
 }
 
 template <int W>
-void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp, bool paired) {
+void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp, bool paired, bool stores) {
     const int Jg = rows_per_group(rp, pl);
     pl.slab.ensure(row_slab_bytes(pl) * Jg);
     pl.halo.ensure(row_halo_bytes(rp.rg) * Jg);
@@ -1577,7 +1580,7 @@ void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp
         TimedScope ts(cx, "y_expand");
         hipLaunchKernelGGL((k_rows_expand<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
                            pl.slab.as<float2>(), pl.halo.as<float2>(), rp.count.as<unsigned>(), rp.wy.as<float>(),
-                           rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2], rp.rg, j0, nrows, rp.dir);
+                           rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2], rp.rg, j0, nrows, rp.dir, stores ? 0 : 1);
         check_launch();
     }
 }
@@ -1607,9 +1610,14 @@ void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, 
     const int nc = half ? 3 : 1;
     if (half && use_rows(g, pl)) {
         RowPlan &rp = get_rowplan(cx, g, pl);
-        LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * 3, cx.stream));
+        // One row group whose y window does not wrap writes every plane of the window
+        // once, and the transforms that follow read only the window: no zeroing.
+        const bool stores = rows_per_group(rp, pl) >= rp.Jn && rp.ly > 0;
+        if (!stores)
+            LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * 3,
+                                   cx.stream));
         TimedScope ts(cx, "spread");
-        LAM_WIDTH_SWITCH(pl.w, (spread_rows_w<W>(cx, g, pl, c, rp, paired)));
+        LAM_WIDTH_SWITCH(pl.w, (spread_rows_w<W>(cx, g, pl, c, rp, paired, stores)));
         return;
     }
     const BinParams bp = bin_params(cx, g, pl, nc);
