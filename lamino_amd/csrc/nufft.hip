@@ -822,14 +822,18 @@ __device__ __forceinline__ Box2 box2(unsigned jt, int j0, const RowGeom &rg) {
 }
 
 // Points per (row, tile) average ~140 at the benchmark shape: 128 threads per
-// workgroup (measured: 0.47 ms per launch at 256 threads, 0.43 at 128)
+// workgroup (measured: 0.47 ms per launch at 256 threads, 0.43 at 128).  The
+// occupancy hint has the compiler fit the kernel in fewer registers (W = 6: 63
+// VGPRs instead of 112, 8 waves per SIMD, which the LDS allows): 0.343 -> 0.290
+// ms.  Width 4 takes a hint of 5 and widths from 12 none, where 6 would spill.
 constexpr int kRowsInterpThreads = 128;
+constexpr int rows_interp_waves(int W) { return W == 4 ? 5 : W >= 12 ? 1 : 6; }
 
 // type 2, step 2: the chunk's 2-D box of its row's slab is staged in LDS
 // ([ncell][3]); each point gathers W^2 x 3 values and writes sum_i d_i c_i and
 // its mirror.
 template <int W>
-__global__ __launch_bounds__(kRowsInterpThreads) void k_rows_interp(const float2 *__restrict__ slab, float2 *__restrict__ c,
+__global__ __launch_bounds__(kRowsInterpThreads, rows_interp_waves(W)) void k_rows_interp(const float2 *__restrict__ slab, float2 *__restrict__ c,
                                                      const Geo *__restrict__ geo, long long P, int R, int C,
                                                      int nfz, int nfx, float beta, PointFrame fr, RowGeom rg,
                                                      int j0, const unsigned *__restrict__ ids,
