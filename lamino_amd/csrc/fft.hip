@@ -15,15 +15,25 @@ namespace {
 // Workgroup size; with register radices up to 12 a stage keeps most of the 256
 // threads busy (512 measured no faster).
 constexpr int kThreads = 256;
-// The workgroup's data tile: L * m <= 3840 complex (30 KiB) plus its line-base
-// tables (2 x L x 8 B <= 2 KiB).  Measured: small tiles with several workgroups
-// per CU hide the load/stage/store latency better than fewer, longer tiles.
-constexpr int kLdsComplex = 3840;
+// The workgroup's data tile: L * m <= 3072 complex (24 KiB) plus its line-base
+// tables (2 x L x 8 B) and a Rader plan's slot tables.  Small tiles with several
+// workgroups per CU hide the load/stage/store latency better than fewer, longer
+// ones; with kFftWaves below, 6 workgroups of 4 waves share a CU (measured: 0.074
+// ms average pass at 30 KiB and 5 waves, 0.068 here, 0.070 at 20 KiB and 7 waves,
+// 0.079 at 40 KiB, 0.113 at 60 KiB).
+constexpr int kLdsComplex = 3072;
+// Longest transform.  Lines longer than kLdsComplex run one per workgroup in a
+// second instantiation of the pass whose per-thread butterfly slots are sized
+// for kMaxLen (and that carries no occupancy hint: it would spill).
+constexpr int kMaxLen = 3840;
+// Occupancy hint (waves per SIMD): the pass fits in 80 VGPRs without spills at
+// this tile size and kLoadBatch, instead of 95.
+constexpr int kFftWaves = 6;
 constexpr int kMaxLines = 128;
 
 // Elements per thread whose global loads (load phase) or LDS reads (store
-// phase) are issued together.
-constexpr int kLoadBatch = 8;
+// phase) are issued together (8 measured the same and needs more registers).
+constexpr int kLoadBatch = 4;
 
 // Lines per tile along the inner (contiguous) index, LDS permitting: measured
 // 16 -> 48 took the 90-point passes of the benchmark grid from 94 to 76 us.
@@ -220,11 +230,11 @@ __device__ __forceinline__ void dft(float2 *v) {
 }
 
 // One radix-R Stockham stage over a (sub-)line set of length len.
-template <int R, int INNER, int NT>
+template <int R, int INNER, int NT, int MAXLEN>
 __device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P, const SubLines &g, bool sub, int len,
                                       int Ns, int sign, const float2 *__restrict__ tw, const FastDiv &fNbf,
                                       const FastDiv &fNs) {
-    constexpr int MAXB = (kLdsComplex / R + NT - 1) / NT;
+    constexpr int MAXB = (MAXLEN / R + NT - 1) / NT;
     const int nbf = len / R;
     const int tot = g.nv * nbf;
     float2 v[MAXB][R];
@@ -310,8 +320,8 @@ __device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, i
     return true;
 }
 
-template <int INNER, int NT>
-__global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
+template <int INNER, int NT, int MAXLEN, int WAVES>
+__global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
     extern __shared__ float2 buf[];
     const int L = P.L, M = P.m, N = P.n;
     const AxisPass &a = P.a;
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(NT) void k_fft_axis(KParams P) {
         const float2 *tw = P.tw + P.st_tw[s];
 #define LAM_STAGE(R)                                                                                           \
     case R:                                                                                                    \
-        stage<R, INNER, NT>(buf, P, g, gs != 0, P.st_len[s], P.st_ns[s], P.st_sign[s], tw, P.fNbf[s], P.fNs[s]); \
+        stage<R, INNER, NT, MAXLEN>(buf, P, g, gs != 0, P.st_len[s], P.st_ns[s], P.st_sign[s], tw, P.fNbf[s], P.fNs[s]); \
         break;
         switch (P.rad[s]) {
             LAM_STAGE(2) LAM_STAGE(3) LAM_STAGE(4) LAM_STAGE(5) LAM_STAGE(6) LAM_STAGE(8) LAM_STAGE(9)
@@ -711,7 +721,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         plan->m = n;
         plan->nrad = nrad;
         for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
-    } else if (n <= kLdsComplex && rader_split(n, S, Q)) {
+    } else if (n <= kMaxLen && rader_split(n, S, Q)) {
         plan->rader = true;
         plan->m = n;
         plan->S = S;
@@ -738,7 +748,7 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
     }
     const int m = plan->m;
-    LAM_REQUIRE(m <= kLdsComplex, LAM_ERR_ARG, "FFT length " + std::to_string(n) + " exceeds the LDS transform limit");
+    LAM_REQUIRE(m <= kMaxLen, LAM_ERR_ARG, "FFT length " + std::to_string(n) + " exceeds the LDS transform limit");
     const long double twopi = 6.283185307179586476925286766559L;
     if (plan->blue) {
         std::vector<std::complex<double>> ch(n), b(m, 0.0), bh(m);
@@ -850,10 +860,17 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L +
                        (plan.rader ? 2 * sizeof(short) * (size_t)plan.n : 0);
     TimedScope ts(cx, "fft_axis");
-    if (inner)
-        hipLaunchKernelGGL((k_fft_axis<1, kThreads>), grid, dim3(kThreads), lds, cx.stream, P);
-    else
-        hipLaunchKernelGGL((k_fft_axis<0, kThreads>), grid, dim3(kThreads), lds, cx.stream, P);
+    if (plan.m <= kLdsComplex) {
+        if (inner)
+            hipLaunchKernelGGL((k_fft_axis<1, kThreads, kLdsComplex, kFftWaves>), grid, dim3(kThreads), lds, cx.stream, P);
+        else
+            hipLaunchKernelGGL((k_fft_axis<0, kThreads, kLdsComplex, kFftWaves>), grid, dim3(kThreads), lds, cx.stream, P);
+    } else {
+        if (inner)
+            hipLaunchKernelGGL((k_fft_axis<1, kThreads, kMaxLen, 1>), grid, dim3(kThreads), lds, cx.stream, P);
+        else
+            hipLaunchKernelGGL((k_fft_axis<0, kThreads, kMaxLen, 1>), grid, dim3(kThreads), lds, cx.stream, P);
+    }
     check_launch();
 }
 
