@@ -198,7 +198,8 @@ typedef struct {
     int method;             /* 1 MBIR1 (CG), 2 MBIR2 (split-Bregman TV), 3 MBIR3 (NAG + qGGMRF) */
     int64_t max_iters;      /* ReconParams::maxIters */
     int64_t inner_iters;    /* ReconParams::innerIters */
-    double lambda, mu, sigma, p, tol, xtol;
+    double lambda, mu, sigma, p, tol, xtol;   /* MBIR3 with mu <= 0: mu is the normal operator's
+                                                  Lipschitz constant, estimated on the device */
     double pad_factor;      /* ReconParams::PAD_FACTOR (1.4142) */
     double eps;             /* NUFFT tolerance for the backprojection (<= 0: default) */
 } lam_recon_params;
@@ -209,6 +210,7 @@ typedef struct {
     int64_t normal_ops;     /* normal-operator applications */
     int64_t cg_iters;       /* inner CG iterations executed */
     int64_t outer_iters;    /* outer iterations executed */
+    double lipschitz;       /* MBIR3: the Lipschitz constant passed to nagopt (mu, or the estimate) */
 } lam_recon_stats;
 
 /* MBIR1/2/3 (include/tomocam.h:101-154; src/mbir_cg.cpp, src/mbir_sb.cpp, src/mbir3.cpp).

@@ -5,7 +5,8 @@
  * through the C ABI of lamino.h.  This header owns the per-thread library
  * context, maps C status codes back to the exceptions the reference throws,
  * and moves host arrays to and from device memory (float32 on the device;
- * double host arrays are converted on the way).
+ * double host arrays are refused unless LAMINO_FP32_FOR_DOUBLE=1 accepts
+ * FP32 arithmetic, and are then converted on the way).
  *
  * Link with -llamino_amd (lamino_amd/liblamino_amd.so).
  */
@@ -76,6 +77,19 @@ namespace tomocam::device {
     template <typename T> struct dev_elem { using type = float; };
     template <typename R> struct dev_elem<std::complex<R>> { using type = std::complex<float>; };
 
+    // The device path computes in FP32 with an ES kernel of width 6 (eps 1e-5),
+    // while the reference's double instantiations run FINUFFT at 1e-14
+    // (include/finufft_plan.h:42).  A double host array is therefore refused
+    // unless the caller accepts FP32 arithmetic by setting
+    // LAMINO_FP32_FOR_DOUBLE=1; the results then come back converted to double.
+    inline void require_fp32_opt_in() {
+        const char *v = std::getenv("LAMINO_FP32_FOR_DOUBLE");
+        if (!v || std::strcmp(v, "1") != 0)
+            throw std::invalid_argument(
+                "lamino: double-precision arrays are computed in FP32 on the device (eps 1e-5, not the "
+                "reference's 1e-14); set LAMINO_FP32_FOR_DOUBLE=1 to accept this");
+    }
+
     // Move-only device buffer of n elements of D, freed on destruction.
     template <typename D>
     class Buffer {
@@ -109,6 +123,7 @@ namespace tomocam::device {
         // Host -> device, converting the element type when the host is double.
         template <typename H>
         void upload(const H *src) {
+            if constexpr (sizeof(H) != sizeof(D)) require_fp32_opt_in();
             if constexpr (sizeof(H) == sizeof(D)) {
                 check(lam_upload(ctx(), p_, src, n_ * sizeof(D)));
             } else {

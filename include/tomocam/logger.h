@@ -1,14 +1,14 @@
 /*
- * tomocam/logger.h -- the progress logger that nagopt accepts
- * (include/logger.h:28-72 of the reference).  The device solvers do not
- * print; a Logger passed to them is accepted and left unused.
+ * tomocam/logger.h -- the type behind nagopt's `Logger *` parameter
+ * (declared at include/optimize.h:129 of the reference).  The device
+ * solvers report nothing while they iterate, so a Logger here is only a
+ * sink that callers may construct and pass; it forwards text to stdout or
+ * appends it to a file and is otherwise unused by this library.
  */
 #ifndef LOGGER_H
 #define LOGGER_H
 
-#include <fstream>
-#include <iostream>
-#include <memory>
+#include <cstdio>
 #include <string>
 
 namespace tomocam {
@@ -17,23 +17,21 @@ namespace tomocam {
 
     class Logger {
         LogMode mode_;
-        std::unique_ptr<std::ofstream> out_;
+        std::string path_;
 
       public:
-        explicit Logger(LogMode mode = LogMode::STDOUT, const std::string &filename = "") : mode_(mode) {
-            if (mode_ == LogMode::FILE) {
-                out_ = std::make_unique<std::ofstream>(filename);
-                if (!out_->is_open()) {
-                    std::cerr << "Warning: Unable to open log file '" << filename << "'. Falling back to STDOUT.\n";
-                    mode_ = LogMode::STDOUT;
+        explicit Logger(LogMode mode = LogMode::STDOUT, std::string path = {}) : mode_(mode), path_(std::move(path)) {}
+        LogMode get_mode() const { return mode_; }
+        void log(const std::string &text) const {
+            if (mode_ == LogMode::STDOUT) {
+                std::fputs(text.c_str(), stdout);
+            } else if (mode_ == LogMode::FILE) {
+                if (std::FILE *f = std::fopen(path_.c_str(), "a")) {
+                    std::fputs(text.c_str(), f);
+                    std::fclose(f);
                 }
             }
         }
-        void log(const std::string &msg) {
-            if (mode_ == LogMode::STDOUT) std::cout << msg;
-            else if (mode_ == LogMode::FILE && out_) *out_ << msg << std::flush;
-        }
-        LogMode get_mode() const { return mode_; }
     };
 
 }  // namespace tomocam

@@ -35,7 +35,7 @@ class lam_recon_params(ctypes.Structure):
 
 class lam_recon_stats(ctypes.Structure):
     _fields_ = [("setup_s", ctypes.c_double), ("solve_s", ctypes.c_double), ("normal_ops", c_i64),
-                ("cg_iters", c_i64), ("outer_iters", c_i64)]
+                ("cg_iters", c_i64), ("outer_iters", c_i64), ("lipschitz", ctypes.c_double)]
 
 
 class lam_job_info(ctypes.Structure):

@@ -58,6 +58,9 @@ __global__ __launch_bounds__(256) void k_crop3(const float *__restrict__ in, flo
     }
 }
 
+constexpr long long kAutoLipschitzIters = 20;
+constexpr uint64_t kAutoLipschitzSeed = 0x4c616d696e6f4c31ull;
+
 // n + 2*floor(n*f/2) with f in float (include/padding.h:98, src/mbir_sb.cpp:51-55)
 long long pad_n(long long n, float f) { return n + 2 * ((long long)((float)n * f) / 2); }
 
@@ -135,6 +138,7 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
     CgStats st;
     long long outer = 0;
     long long normal_ops = -1;
+    double lip = 0;
     if (pr.method == 1) {
         OpFn A = [&](CVol3 u, Vol3 y) { toep_apply(*toep, u, y, 0.f); };
         if (mode == MbirMode::Graph || mode == MbirMode::Async) {
@@ -182,8 +186,20 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
                 for (int c = 0; c < 3; ++c) e += std::sqrt(q[3 * g + c] - 2.0 * s[c] + yTy);
             return e;
         };
+        // src/mbir3.cpp:123-125 passes mu as the Lipschitz constant, so the step is 0.1/mu
+        // whatever the operator's norm; that diverges once the normal operator's largest
+        // eigenvalue exceeds 20 mu (3 x 360 projections put it in the thousands).  mu <= 0
+        // asks for the normal operator's constant instead: an FP64 power iteration
+        // (src/lipschitz.cpp:30-53, 20 iterations, a lower bound that converges from below).
+        double L = pr.mu;
+        if (L <= 0) {
+            OpFn N = [&](CVol3 u, Vol3 y) { toep_apply(*toep, u, y, 0.f); };
+            L = lipschitz(cx, od, N, kAutoLipschitzIters, 0.0, kAutoLipschitzSeed);
+            LAM_REQUIRE(L > 0, LAM_ERR_ARG, "MBIR3: the normal operator vanishes (Lipschitz estimate 0)");
+        }
+        lip = L;
         long long calls = 0;
-        nagopt(cx, od, grad, loss, x.v(), pr.max_iters, pr.mu, pr.tol, pr.xtol, pr.inner_iters, &outer, &calls);
+        nagopt(cx, od, grad, loss, x.v(), pr.max_iters, L, pr.tol, pr.xtol, pr.inner_iters, &outer, &calls);
     }
     // crop to the requested dims and download
     const long long m = rd[0] * rd[1] * rd[2];
@@ -203,6 +219,7 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
         stats->normal_ops = normal_ops >= 0 ? normal_ops : toep->applies;
         stats->cg_iters = st.iters;
         stats->outer_iters = outer;
+        stats->lipschitz = lip;
     }
 }
 

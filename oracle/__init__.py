@@ -253,6 +253,121 @@ def mbir(method, datasets, recon_dims, *, max_iters, inner_iters=3, lam=0.1, mu=
     return out
 
 
+# ---------------------------------------------------------------- solvers on Python operators
+_OP_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp)
+_LOSS_CB = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, _dp)
+
+
+def _wrap_op(fn, shape):
+    """ctypes callback y = fn(x) on [3, *shape] double arrays."""
+    n = 3 * int(np.prod(shape))
+
+    def cb(_user, xp, yp):
+        x = np.ctypeslib.as_array(xp, shape=(n,)).reshape((3,) + tuple(shape)).copy()
+        y = np.asarray(fn(x), dtype=np.float64).reshape(-1)
+        np.ctypeslib.as_array(yp, shape=(n,))[:] = y
+
+    return _OP_CB(cb)
+
+
+def _wrap_loss(fn, shape):
+    n = 3 * int(np.prod(shape))
+
+    def cb(_user, xp):
+        x = np.ctypeslib.as_array(xp, shape=(n,)).reshape((3,) + tuple(shape)).copy()
+        return float(fn(x))
+
+    return _LOSS_CB(cb)
+
+
+def cgsolver(A, b, x0, max_iter, tol, prec=64):
+    """src/conjgrad.cpp:36-117 with the operator A(x) given in Python ([3, n1, n2, n3])."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    dd, dp = _l(b.shape[1:])
+    cb = _wrap_op(A, b.shape[1:])
+    out = np.empty_like(b)
+    it = ctypes.c_int(0)
+    _chk(lib().orc_cgsolver_cb(prec, dp, cb, None, _d(b), _d(x0), ctypes.c_longlong(max_iter),
+                               ctypes.c_double(tol), _d(out), ctypes.byref(it)))
+    return out, it.value
+
+
+def split_bregman(A, yT, x0, lam, mu, outer_max, inner_max, tol=0.0, xtol=0.0, prec=64):
+    """src/bregman.cpp:38-134 with the operator A(x) given in Python."""
+    yT = np.ascontiguousarray(yT, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    dd, dp = _l(yT.shape[1:])
+    cb = _wrap_op(A, yT.shape[1:])
+    out = np.empty_like(yT)
+    _chk(lib().orc_split_bregman_cb(prec, dp, cb, None, _d(yT), _d(x0), ctypes.c_double(lam), ctypes.c_double(mu),
+                                    ctypes.c_longlong(outer_max), ctypes.c_longlong(inner_max), ctypes.c_double(tol),
+                                    ctypes.c_double(xtol), _d(out)))
+    return out
+
+
+def nagopt(grad, loss, x, max_iters, lipschitz, tol=0.0, xtol=0.0, max_inner=3, prec=64):
+    """src/nagopt.cpp:115-209 with grad(x) and loss(x) given in Python; returns the result
+    (the reference also mutates x; the input array here is left alone)."""
+    x = np.array(x, dtype=np.float64, copy=True)
+    dd, dp = _l(x.shape[1:])
+    gcb = _wrap_op(grad, x.shape[1:])
+    lcb = _wrap_loss(loss, x.shape[1:])
+    _chk(lib().orc_nagopt_cb(prec, dp, gcb, lcb, None, _d(x), ctypes.c_longlong(max_iters),
+                             ctypes.c_double(lipschitz), ctypes.c_double(tol), ctypes.c_double(xtol),
+                             ctypes.c_longlong(max_inner)))
+    return x
+
+
+def lipschitz(A, x0, max_iter=20, tol=1e-6, prec=64):
+    """src/lipschitz.cpp:30-53 (power iteration) over the 3-component vector, from start x0."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    dd, dp = _l(x0.shape[1:])
+    cb = _wrap_op(A, x0.shape[1:])
+    L = ctypes.c_double(0)
+    _chk(lib().orc_lipschitz_cb(prec, dp, cb, None, _d(x0), ctypes.c_longlong(max_iter), ctypes.c_double(tol),
+                                ctypes.byref(L)))
+    return L.value
+
+
+def _pack_datasets(datasets):
+    Ps = np.array([d[0].shape[0] for d in datasets], dtype=np.int64)
+    Rs = np.array([d[0].shape[1] for d in datasets], dtype=np.int64)
+    Cs = np.array([d[0].shape[2] for d in datasets], dtype=np.int64)
+    proj = np.concatenate([np.ascontiguousarray(d[0], dtype=np.float64).ravel() for d in datasets])
+    al = np.concatenate([_angles(d[1], d[2], int(P))[0] for d, P in zip(datasets, Ps)])
+    ga = np.concatenate([_angles(d[1], d[2], int(P))[1] for d, P in zip(datasets, Ps)])
+    return Ps, Rs, Cs, proj, al, ga
+
+
+def mbir_setup(method, datasets, recon_dims, eps=1e-9, prec=64):
+    """The drivers' shared set-up: (padded dims, yT [3, *padded])."""
+    Ps, Rs, Cs, proj, al, ga = _pack_datasets(datasets)
+    dd, dp = _l(recon_dims)
+    od = np.zeros(3, np.int64)
+    _chk(lib().orc_mbir_setup(prec, ctypes.c_double(eps), method, dp, len(datasets), Ps.ctypes.data_as(_lp),
+                              Rs.ctypes.data_as(_lp), Cs.ctypes.data_as(_lp), _d(proj), _d(al), _d(ga),
+                              od.ctypes.data_as(_lp), None))
+    yT = np.empty((3,) + tuple(int(v) for v in od))
+    _chk(lib().orc_mbir_setup(prec, ctypes.c_double(eps), method, dp, len(datasets), Ps.ctypes.data_as(_lp),
+                              Rs.ctypes.data_as(_lp), Cs.ctypes.data_as(_lp), _d(proj), _d(al), _d(ga),
+                              od.ctypes.data_as(_lp), _d(yT)))
+    return tuple(int(v) for v in od), yT
+
+
+def mbir3_closures(datasets, recon_dims, x, sigma=1000.0, p=1.2, eps=1e-9, prec=64):
+    """MBIR3's gradient and loss closures (src/mbir3.cpp:86-117) at the padded iterate x."""
+    Ps, Rs, Cs, proj, al, ga = _pack_datasets(datasets)
+    dd, dp = _l(recon_dims)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.empty_like(x)
+    loss = ctypes.c_double(0)
+    _chk(lib().orc_mbir3_closures(prec, ctypes.c_double(eps), dp, len(datasets), Ps.ctypes.data_as(_lp),
+                                  Rs.ctypes.data_as(_lp), Cs.ctypes.data_as(_lp), _d(proj), _d(al), _d(ga),
+                                  ctypes.c_double(sigma), ctypes.c_double(p), _d(x), _d(g), ctypes.byref(loss)))
+    return g, loss.value
+
+
 # ---------------------------------------------------------------- reference (_ref)
 class Ref:
     """The reference's own compiled sources (oracle/_ref/libref.so)."""

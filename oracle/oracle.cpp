@@ -920,6 +920,35 @@ Vec3<T> nagopt(const Op<T> &grad, const std::function<double(const Vec3<T> &)> &
     return x;
 }
 
+// src/lipschitz.cpp:30-53: power iteration x <- Ax/|Ax|, L = |Ax|, stop when
+// |L_new - L| < tol (the previous estimate is returned then), 0 when |Ax|
+// vanishes.  The reference's domain is one Array; here it is the 3-component
+// vector the vector solvers iterate on (|.| over all three), with the start
+// vector supplied by the caller instead of Array::random_like.
+template <class T>
+T lipschitz(const Op<T> &A, const Vec3<T> &x0, const Dims &d, size_t max_iter, T tol) {
+    auto norm3 = [&](const Vec3<T> &v) {
+        double s = 0;
+        for (int i = 0; i < 3; ++i) s += dot(v[i], v[i]);
+        return (T)std::sqrt(s);
+    };
+    Vec3<T> x = x0;
+    T nx = norm3(x);
+    for (int i = 0; i < 3; ++i)
+        for (size_t q = 0; q < d.size(); ++q) x[i][q] /= nx;
+    T L = 0;
+    for (size_t it = 0; it < max_iter; ++it) {
+        Vec3<T> Ax = A(x);
+        T na = norm3(Ax);
+        if (std::abs(na) < (T)1e-12) { L = 0; break; }
+        for (int i = 0; i < 3; ++i)
+            for (size_t q = 0; q < d.size(); ++q) x[i][q] = Ax[i][q] / na;
+        if (std::abs(na - L) < tol) break;
+        L = na;
+    }
+    return L;
+}
+
 // --------------------------------------------------------------------------
 // MBIR drivers (src/mbir_cg.cpp, src/mbir_sb.cpp, src/mbir3.cpp)
 // --------------------------------------------------------------------------
@@ -940,8 +969,18 @@ struct Params {
 // pad rule n + 2*floor(n*f/2) with the float arithmetic of include/padding.h:98
 template <class T> static size_t pad_n(size_t n, T f) { return n + 2 * ((size_t)(n * f) / 2); }
 
+// The job set-up every driver shares (src/mbir_cg.cpp:53-97, src/mbir_sb.cpp:51-92,
+// src/mbir3.cpp:51-92): padded volume dims, one grid per dataset (projections
+// normalised by their max and pad2d-ed), yT = sum of the datasets' adjoints.
+template <class T> struct Job {
+    Dims od;
+    std::vector<Grid<T>> grids;
+    Vec3<T> yT;
+};
+
 template <class T>
-Vec3<T> mbir(const Ctx &cx, const std::vector<Dataset> &ds, const Dims &rd, const Params &pr) {
+Job<T> mbir_setup(const Ctx &cx, const std::vector<Dataset> &ds, const Dims &rd, const Params &pr) {
+    Job<T> job;
     T padding = pr.method == 1 ? (T)((float)pr.pad_factor - 1.0) : (T)(float)pr.pad_factor;
     T vpad = pr.method == 1 ? padding : (T)(padding - 1);
     Dims od{pad_n(rd.n1, vpad), pad_n(rd.n2, vpad), pad_n(rd.n3, vpad)};
@@ -966,6 +1005,46 @@ Vec3<T> mbir(const Ctx &cx, const std::vector<Dataset> &ds, const Dims &rd, cons
         for (int i = 0; i < 3; ++i)
             for (size_t q = 0; q < od.size(); ++q) yT[i][q] += a[i][q];
     }
+    job.od = od;
+    job.grids = std::move(grids);
+    job.yT = std::move(yT);
+    return job;
+}
+
+// src/mbir3.cpp:86-117: the gradient closure (per dataset sysmat(x) - yT, so
+// the summed yT is subtracted G times, then the qGGMRF prior) and the loss
+// closure (per dataset and component sqrt(x.Ax - 2 x.yT + yTy)).
+template <class T> Vec3<T> mbir3_grad(const Ctx &cx, const Job<T> &job, const Vec3<T> &x, const Params &pr) {
+    const Dims &od = job.od;
+    Vec3<T> g;
+    for (size_t gi = 0; gi < job.grids.size(); ++gi) {
+        Vec3<T> t = sysmat(cx, x, od, job.grids[gi]);
+        for (int i = 0; i < 3; ++i) {
+            if (gi == 0) g[i].assign(od.size(), 0);
+            for (size_t q = 0; q < od.size(); ++q) g[i][q] += t[i][q] - job.yT[i][q];
+        }
+    }
+    for (int i = 0; i < 3; ++i) qggmrf<T>(g[i], x[i], od, (T)pr.sigma, (T)pr.p);
+    return g;
+}
+
+template <class T> double mbir3_loss(const Ctx &cx, const Job<T> &job, const Vec3<T> &x) {
+    double yTy = 0;
+    for (int i = 0; i < 3; ++i) yTy += dot(job.yT[i], job.yT[i]);
+    double e = 0;
+    for (size_t gi = 0; gi < job.grids.size(); ++gi) {
+        Vec3<T> t = sysmat(cx, x, job.od, job.grids[gi]);
+        for (int i = 0; i < 3; ++i) e += std::sqrt(dot(x[i], t[i]) - 2.0 * dot(x[i], job.yT[i]) + yTy);
+    }
+    return e;
+}
+
+template <class T>
+Vec3<T> mbir(const Ctx &cx, const std::vector<Dataset> &ds, const Dims &rd, const Params &pr) {
+    Job<T> job = mbir_setup<T>(cx, ds, rd, pr);
+    const Dims od = job.od;
+    const auto &grids = job.grids;
+    const Vec3<T> &yT = job.yT;
     Op<T> A = [&](const Vec3<T> &x) {
         Vec3<T> Ax = sysmat(cx, x, od, grids[0]);
         for (size_t g = 1; g < grids.size(); ++g) {
@@ -984,28 +1063,8 @@ Vec3<T> mbir(const Ctx &cx, const std::vector<Dataset> &ds, const Dims &rd, cons
         rec = split_bregman<T>(A, yT, x0, od, (T)pr.lambda, (T)pr.mu, pr.max_iters, pr.inner_iters, (T)pr.tol,
                                (T)pr.xtol);
     } else {
-        Op<T> grad = [&](const Vec3<T> &x) {
-            Vec3<T> g;
-            for (size_t gi = 0; gi < grids.size(); ++gi) {
-                Vec3<T> t = sysmat(cx, x, od, grids[gi]);
-                for (int i = 0; i < 3; ++i) {
-                    if (gi == 0) g[i].assign(od.size(), 0);
-                    for (size_t q = 0; q < od.size(); ++q) g[i][q] += t[i][q] - yT[i][q];
-                }
-            }
-            for (int i = 0; i < 3; ++i) qggmrf<T>(g[i], x[i], od, (T)pr.sigma, (T)pr.p);
-            return g;
-        };
-        double yTy = 0;
-        for (int i = 0; i < 3; ++i) yTy += dot(yT[i], yT[i]);
-        auto loss = [&](const Vec3<T> &x) {
-            double e = 0;
-            for (size_t gi = 0; gi < grids.size(); ++gi) {
-                Vec3<T> t = sysmat(cx, x, od, grids[gi]);
-                for (int i = 0; i < 3; ++i) e += std::sqrt(dot(x[i], t[i]) - 2.0 * dot(x[i], yT[i]) + yTy);
-            }
-            return e;
-        };
+        Op<T> grad = [&](const Vec3<T> &x) { return mbir3_grad<T>(cx, job, x, pr); };
+        auto loss = [&](const Vec3<T> &x) { return mbir3_loss<T>(cx, job, x); };
         rec = nagopt<T>(grad, loss, x0, od, pr.max_iters, (T)pr.mu, (T)pr.tol, (T)pr.xtol, pr.inner_iters);
     }
     // crop3d symmetric (include/padding.h:206-232)
@@ -1267,6 +1326,149 @@ int orc_cg_diag(int prec, const long long *dims, const double *diag3, const doub
             };
             Vec3<T> x = cgsolver<T>(A, b, x0, d, (size_t)max_iter, (T)tol, iters);
             for (int i = 0; i < 3; ++i) fromvec(x[i], x3 + i * d.size());
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+// Solvers on caller-supplied operators (ctypes callbacks on double arrays
+// [3][n]); the oracle's arithmetic runs in T and converts at the callback.
+typedef void (*orc_op_cb)(void *user, const double *x3, double *y3);
+typedef double (*orc_loss_cb)(void *user, const double *x3);
+}  // extern "C"
+
+namespace {
+template <class T> Vec3<T> load3(const double *a, size_t n) {
+    Vec3<T> v;
+    for (int i = 0; i < 3; ++i) v[i] = tovec<T>(a + i * n, n);
+    return v;
+}
+template <class T> void store3(const Vec3<T> &v, double *a) {
+    for (int i = 0; i < 3; ++i) fromvec(v[i], a + i * v[i].size());
+}
+template <class T> Op<T> cb_op(orc_op_cb cb, void *user, size_t n) {
+    return [=](const Vec3<T> &x) {
+        std::vector<double> in(3 * n), out(3 * n);
+        store3(x, in.data());
+        cb(user, in.data(), out.data());
+        return load3<T>(out.data(), n);
+    };
+}
+std::vector<Dataset> mkds(int G, const long long *P, const long long *R, const long long *C, const double *proj,
+                          const double *alpha, const double *gamma) {
+    std::vector<Dataset> ds;
+    size_t po = 0, ao = 0;
+    for (int g = 0; g < G; ++g) {
+        ds.push_back(Dataset{(size_t)P[g], (size_t)R[g], (size_t)C[g], proj + po, alpha + ao, gamma + ao});
+        po += (size_t)(P[g] * R[g] * C[g]);
+        ao += (size_t)P[g];
+    }
+    return ds;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_split_bregman_cb(int prec, const long long *dims, orc_op_cb A, void *user, const double *yT3,
+                         const double *x03, double lambda, double mu, long long outer_max, long long inner_max,
+                         double tol, double xtol, double *out3) {
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Dims d = mkd(dims);
+            Vec3<T> x = split_bregman<T>(cb_op<T>(A, user, d.size()), load3<T>(yT3, d.size()),
+                                         load3<T>(x03, d.size()), d, (T)lambda, (T)mu, (size_t)outer_max,
+                                         (size_t)inner_max, (T)tol, (T)xtol);
+            store3(x, out3);
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+int orc_cgsolver_cb(int prec, const long long *dims, orc_op_cb A, void *user, const double *b3, const double *x03,
+                    long long max_iter, double tol, double *out3, int *iters) {
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Dims d = mkd(dims);
+            Vec3<T> x = cgsolver<T>(cb_op<T>(A, user, d.size()), load3<T>(b3, d.size()), load3<T>(x03, d.size()),
+                                    d, (size_t)max_iter, (T)tol, iters);
+            store3(x, out3);
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+// x3 is the caller's x, updated in place as the reference's nagopt mutates it.
+int orc_nagopt_cb(int prec, const long long *dims, orc_op_cb grad, orc_loss_cb loss, void *user, double *x3,
+                  long long max_iters, double lipschitz, double tol, double xtol, long long max_inner) {
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Dims d = mkd(dims);
+            const size_t n = d.size();
+            Vec3<T> x = load3<T>(x3, n);
+            auto lf = [=](const Vec3<T> &v) {
+                std::vector<double> in(3 * n);
+                store3(v, in.data());
+                return loss(user, in.data());
+            };
+            Vec3<T> r = nagopt<T>(cb_op<T>(grad, user, n), lf, x, d, (size_t)max_iters, (T)lipschitz, (T)tol,
+                                  (T)xtol, (size_t)max_inner);
+            store3(r, x3);
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+int orc_lipschitz_cb(int prec, const long long *dims, orc_op_cb A, void *user, const double *x03,
+                     long long max_iter, double tol, double *L) {
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Dims d = mkd(dims);
+            *L = (double)lipschitz<T>(cb_op<T>(A, user, d.size()), load3<T>(x03, d.size()), d, (size_t)max_iter,
+                                      (T)tol);
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+// The MBIR drivers' shared set-up: padded dims (od3) and yT [3][od].  Call
+// with yT3 = nullptr to get the dims only.
+int orc_mbir_setup(int prec, double eps, int method, const long long *recon_dims, int G, const long long *P,
+                   const long long *R, const long long *C, const double *proj, const double *alpha,
+                   const double *gamma, long long *od3, double *yT3) {
+    return guard([&] {
+        Params pr;
+        pr.method = method;
+        Ctx cx; cx.eps = eps;
+        auto ds = mkds(G, P, R, C, proj, alpha, gamma);
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Job<T> job = mbir_setup<T>(cx, ds, mkd(recon_dims), pr);
+            od3[0] = job.od.n1; od3[1] = job.od.n2; od3[2] = job.od.n3;
+            if (yT3) store3(job.yT, yT3);
+        };
+        if (prec == 32) run(float{}); else run(double{});
+    });
+}
+
+// MBIR3's gradient and loss closures at a padded iterate x3.
+int orc_mbir3_closures(int prec, double eps, const long long *recon_dims, int G, const long long *P,
+                       const long long *R, const long long *C, const double *proj, const double *alpha,
+                       const double *gamma, double sigma, double p, const double *x3, double *g3, double *loss) {
+    return guard([&] {
+        Params pr;
+        pr.method = 3; pr.sigma = sigma; pr.p = p;
+        Ctx cx; cx.eps = eps;
+        auto ds = mkds(G, P, R, C, proj, alpha, gamma);
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Job<T> job = mbir_setup<T>(cx, ds, mkd(recon_dims), pr);
+            Vec3<T> x = load3<T>(x3, job.od.size());
+            store3(mbir3_grad<T>(cx, job, x, pr), g3);
+            *loss = mbir3_loss<T>(cx, job, x);
         };
         if (prec == 32) run(float{}); else run(double{});
     });

@@ -154,3 +154,41 @@ def test_autotuned_tiles_keep_projector_parity(ctx, orc):
     y1 = la.forward(m, pg)
     assert rel(a1, a0) < 1e-5 and rel(y1, y0) < 1e-5
     assert rel(a1, orc.adjoint(proj, dims, al, ga)) < 1e-4
+
+
+@pytest.mark.parametrize("cls", ["[16,64,64] MBIR1 1x90", "[32,128,128] MBIR2 2x180",
+                                 "[51,256,256] MBIR3 3x360", "[51,511,511] MBIR2 2x360"])
+def test_full_size_job_per_volume_class_graph_equals_plain(ctx, cls):
+    """One job of each volume class of the batched and served mixes at its real shape (the
+    largest: volume [51,511,511] -> padded (721,71,721), detector 511 -> 1233) through the
+    graph-form server: equal to the plain form, finite, and the measured device peak within
+    the admission estimate.  Iterations are cut to a few; the shapes are what is tested."""
+    import lamino_amd as la
+    from lamino_amd import synth
+    from lamino_amd.serve import Server
+
+    vol, meth, gp = cls.split()
+    v = [int(t) for t in vol.strip("[]").split(",")]
+    G, P = (int(t) for t in gp.split("x"))
+    method = int(meth[-1])
+    params = {1: dict(max_iters=5, tol=0.0),
+              2: dict(lam=0.1, mu=10.0, max_iters=2, inner_iters=3, tol=0.0, xtol=0.0),
+              3: dict(sigma=1000.0, p=1.2, mu=0.0, max_iters=2, inner_iters=2, tol=0.0, xtol=0.0)}[method]
+    work = synth.Workload(cls, method, synth.recon_dims_internal(v), synth.oddify(v[1]),
+                          [synth.lamino_scan(P, 30.0, 45.0 * g) for g in range(G)], synth.SEED_BASE + 77, noise=0.01,
+                          params=params)
+
+    def project(mp, scan, dp):
+        return la.forward(mp, la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp))
+
+    ds, _ = synth.simulate(work, project)
+    prm = synth.recon_params(work)
+    say(f"{cls}: solver shapes {work.solver_shapes()}")
+    plain = la.api._mbir(method, ds, work.recon_dims, prm)
+    with Server(slots=1, mode="graph") as s:
+        x, info = s.wait(s.submit(method, ds, work.recon_dims, prm))
+    say(f"{cls}: graph vs plain {rel(x, plain):.2e}, peak {info['footprint_peak'] / 2**30:.2f} GiB, "
+        f"est {info['footprint_est'] / 2**30:.2f} GiB, stats {info['stats']}")
+    assert np.isfinite(plain).all() and np.isfinite(x).all()
+    assert rel(x, plain) < MIX_TOL
+    assert info["footprint_peak"] <= info["footprint_est"]
