@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the laminography reconstruction hot path on one MI355X.
 
-Workload (split-Bregman configuration): volume [32,128,128] x 3 -> padded
-(179, 43, 179); detector 128 -> 127 -> padded 305 x 305; two gamma datasets
-(0, 45 deg) of 180 projections; tilt 35 deg.  A step is one forward plus one
-adjoint projection of every dataset (all 3 magnetisation components), with
-the inputs resident in HBM.  Also reported: CG iterations/s of the Toeplitz
-normal operator + ramp preconditioner at the same shape, the plain-form
-reconstruction latency of the whole split-Bregman job (30 outer x 3 inner),
-the roofline of the dominant kernel and the CPU oracle timed on this host.
+Headline workload (precision-study configuration, the largest single-GPU one):
+volume [51,256,256] x 3 -> padded (359, 71, 359); detector 256 -> 255 ->
+padded 615 x 615; three gamma datasets (0, 45, 90 deg) of 360 projections;
+tilt 30 deg.  A step is one forward plus one adjoint projection of every
+dataset (all 3 magnetisation components), inputs resident in HBM.  The point
+binning of each grid is built once (warmup) and reused by every step; the
+reference re-sorts the points in every call.
+
+Also reported: the plain-CG and split-Bregman configurations' throughput
+(extra keys), CG iterations/s of the Toeplitz normal operator + ramp
+preconditioner at each shape, the plain-form reconstruction latency of each
+configuration's job, the batched and served forms, the roofline of the
+dominant kernel and the CPU oracle timed on this host.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -27,6 +31,8 @@ sys.path.insert(0, HERE)
 
 METRIC = "fwd+adj NUFFT projections/s, CG iters/s vs plain form; p50/p99 recon latency"
 HBM_PEAK_GBS = 8000.0
+KERNEL_SCOPES = ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points", "spread_rows2d",
+                 "y_expand", "y_contract", "interp_rows2d", "toep_mul")
 
 
 def parse():
@@ -34,42 +40,261 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3),
+                    help="headline configuration (1 plain CG, 2 split-Bregman, 3 precision study)")
+    ap.add_argument("--extra-configs", default="1,2", help="configurations also measured (extra keys)")
     ap.add_argument("--cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-recon", action="store_true", help="skip the full reconstruction latency")
-    ap.add_argument("--cpu-projections", type=int, default=24)
+    ap.add_argument("--no-recon", action="store_true", help="skip the reconstruction latencies")
+    ap.add_argument("--cpu-projections", type=int, default=0, help="CPU sample size (0: per configuration)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (16 uneven jobs) measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the served (Poisson arrivals) measurement")
     ap.add_argument("--batched-jobs", type=int, default=16)
     ap.add_argument("--served-jobs", type=int, default=12)
     ap.add_argument("--slots", type=int, default=4, help="concurrent jobs of the batched/served server")
     ap.add_argument("--profile", action="store_true",
-                    help="only the timed fwd+adj steps plus a 1 GiB device-to-device copy (PMC calibration), "
-                         "for rocprofv3 runs")
+                    help="only the headline's timed fwd+adj steps, a few CG iterations and a 1 GiB "
+                         "device-to-device copy (PMC calibration), for rocprofv3 runs")
     return ap.parse_args()
-
-
-def hbm_traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, when present."""
-    path = os.path.join(HERE, "profiles", "hbm_traffic.json")
-    if not os.path.exists(path):
-        return None
-    try:
-        data = json.load(open(path))
-        return data.get(kernel, {}).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
-        return None
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def hbm_traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` at `workload` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        ent = data.get(workload, {}).get(kernel, {})
+        return ent.get("hbm_bytes_per_launch")
+    except (OSError, ValueError, AttributeError):
+        return None
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def smooth_even(n):
+    while True:
+        r = n
+        for p_ in (2, 3, 5):
+            while r % p_ == 0:
+                r //= p_
+        if r == 1 and n % 2 == 0:
+            return n
+        n += 1
+
+
+def _rel(a, b):
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+class Projectors:
+    """One configuration's forward + adjoint step with everything resident on the device."""
+
+    def __init__(self, la, work):
+        from lamino_amd import synth
+
+        self.la, self.work = la, work
+        self.vol, self.det = work.solver_shapes()
+        self.G = len(work.scans)
+        self.P = len(work.scans[0].alpha)
+        self.ctx = la.default_context()
+        m = np.zeros((3,) + self.vol, np.float32)
+        rd = work.recon_dims
+        o = [(v - n) // 2 for v, n in zip(self.vol, rd)]
+        m[:, o[0]:o[0] + rd[0], o[1]:o[1] + rd[1], o[2]:o[2] + rd[2]] = synth.vortex_field(rd, work.seed)
+        self.m = m
+        self.md = self.ctx.to_device(m)
+        self.grids = [la.PolarGrid.per_projection(s.alpha, s.gamma, self.det, self.det) for s in work.scans]
+        self.projs = [self.ctx.alloc((self.P, self.det, self.det), np.float32) for _ in range(self.G)]
+        self.outs = [self.ctx.alloc((3,) + self.vol, np.float32) for _ in range(self.G)]
+
+    def step(self):
+        from lamino_amd import _lib
+        from lamino_amd.api import _dims
+
+        lib, ctx, eps = self.ctx.lib, self.ctx, self.la.api.DEFAULT_EPS
+        dims = _dims(self.vol)
+        for g in range(self.G):
+            a, b, c = self.md.comps()
+            _lib.check(lib.lam_forward(ctx.h, self.grids[g].h, dims, eps, a, b, c, self.projs[g].ptr))
+            x, y, z = self.outs[g].comps()
+            _lib.check(lib.lam_adjoint(ctx.h, self.grids[g].h, dims, eps, self.projs[g].ptr, x, y, z))
+
+    def timed(self, steps, warmup):
+        """(ms per step, spread launch total ms, spread launches): events around the spreader only."""
+        from lamino_amd import _lib
+
+        lib, ctx = self.ctx.lib, self.ctx
+        # the first step builds every grid's point bins (key, bucket scatter, in-bucket order)
+        _lib.check(lib.lam_timing_enable(ctx.h, 1))
+        _lib.check(lib.lam_timing_only(ctx.h, b"bin_points"))
+        self.step()
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(lib.lam_timing_query(ctx.h, b"bin_points", ctypes.byref(tot), ctypes.byref(cnt)))
+        self.bin_ms = {"total_ms": round(tot.value, 3), "builds": cnt.value}
+        for _ in range(max(warmup - 1, 0)):
+            self.step()
+        ctx.sync()
+        # Every timed scope adds two event records to the stream; timing all ~28 launches
+        # of a step costs about 5 % of it, so the timed region records the spreader only.
+        _lib.check(lib.lam_timing_enable(ctx.h, 1))
+        _lib.check(lib.lam_timing_only(ctx.h, b"spread"))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+        ctx.sync()
+        t1 = time.perf_counter()
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(lib.lam_timing_query(ctx.h, b"spread", ctypes.byref(tot), ctypes.byref(cnt)))
+        _lib.check(lib.lam_timing_only(ctx.h, None))
+        _lib.check(lib.lam_timing_enable(ctx.h, 0))
+        return (t1 - t0) * 1e3 / steps, tot.value, cnt.value
+
+    def diagnostic(self, steps):
+        """Per-kernel time per step from a separate pass with every scope timed."""
+        from lamino_amd import _lib
+
+        lib, ctx = self.ctx.lib, self.ctx
+        _lib.check(lib.lam_timing_only(ctx.h, None))
+        _lib.check(lib.lam_timing_enable(ctx.h, 1))   # enabling clears earlier records
+        for _ in range(steps):
+            self.step()
+        ctx.sync()
+        kernels = {}
+        for name in KERNEL_SCOPES:
+            tot, cnt = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(lib.lam_timing_query(ctx.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt)))
+            if cnt.value:
+                kernels[name] = {"total_ms": tot.value, "launches": cnt.value, "avg_ms": tot.value / cnt.value}
+        _lib.check(lib.lam_timing_enable(ctx.h, 0))
+        return kernels
+
+    def cg(self, iters):
+        """CG iterations/s of the Toeplitz normal operator (all datasets) + ramp preconditioner."""
+        from lamino_amd import _lib
+        from lamino_amd.api import _dims, _ptrs
+
+        la, ctx, lib = self.la, self.ctx, self.ctx.lib
+        tb0 = time.perf_counter()
+        T = la.ToeplitzOperator(self.grids, self.vol)
+        ctx.sync()
+        build_s = time.perf_counter() - tb0
+        dims = _dims(self.vol)
+        b = self.outs[0]
+        xd = ctx.alloc((3,) + self.vol, np.float32)
+        x0 = ctx.alloc((3,) + self.vol, np.float32)
+        x0.zero()
+        op = _lib.OP_FN(ctypes.cast(lib.lam_toep_op, ctypes.c_void_p).value)
+        it = ctypes.c_int64()
+        _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), 2, 0.0,
+                                    _ptrs(xd.comps()), ctypes.byref(it)))
+        ctx.sync()
+        tc0 = time.perf_counter()
+        _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), iters, 0.0,
+                                    _ptrs(xd.comps()), ctypes.byref(it)))
+        ctx.sync()
+        cg_s = time.perf_counter() - tc0
+        return {"cg_iters_per_s": round(it.value / cg_s, 2), "cg_ms_per_iter": round(1e3 * cg_s / it.value, 4),
+                "toeplitz_build_s": round(build_s, 4)}
+
+    def release(self):
+        self.grids = self.projs = self.outs = self.md = None
+
+
+def roofline(pr, kernels, spread_ms, spread_cnt, workload_key):
+    """Roofline of the dominant NUFFT kernel (the spreader): algorithmic bytes per launch
+    M*(12 + 24) + Nfine*24 (coordinates + 3 complex components per point, fine grid once)
+    over its average launch time measured with HIP events on the library's stream."""
+    vol, det, P = pr.vol, pr.det, pr.P
+    nf = [smooth_even(max(2 * n, 12)) for n in vol]
+    Nfine = nf[0] * nf[1] * nf[2]
+    M = P * det * det
+    bytes_alg = M * 36 + Nfine * 24
+    avg_ms = spread_ms / spread_cnt if spread_cnt else kernels.get("spread", {}).get("avg_ms", float("nan"))
+    achieved = bytes_alg / (avg_ms / 1e3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic_from_profiles("spread", workload_key),
+           "kernel": "spread", "bytes_per_launch": bytes_alg, "avg_launch_ms": round(avg_ms, 4)}
+    # The roofs that bind the row-factorised kernels: per half-plane point the 2-D spreader
+    # read-modify-writes W^2 cells x 3 complex components (2 x 24 B each) in its LDS box and
+    # the 2-D interpolator reads them once (LDS roof; peak = 256 CUs x 128 B/clk x 2.4 GHz,
+    # an estimate from the CU's LDS width, not a published figure); the y passes stream the
+    # row slabs and the fine grid (HBM roof).
+    w = 6
+    half_points = P * ((det * det + 1) // 2)
+    rows = ((det * det + 1) // 2 - 1) // det + 1
+    slab = nf[0] * nf[2] * 24
+    lds_peak_tbs = 256 * 128 * 2.4e9 / 1e12
+    binding = {}
+    for k, per_pt in (("spread_rows2d", 2 * 24 * w ** 2), ("interp_rows2d", 24 * w ** 2)):
+        if k in kernels:
+            n_launch = kernels[k]["launches"] / max(1, kernels["spread" if k.startswith("spread") else "interp"]["launches"])
+            tbs = half_points * per_pt / (kernels[k]["avg_ms"] * n_launch / 1e3) / 1e12
+            binding[k] = {"roof": "lds", "achieved_tbs": round(tbs, 2), "peak_tbs": round(lds_peak_tbs, 1),
+                          "frac": round(tbs / lds_peak_tbs, 3)}
+    for k, nbytes in (("y_contract", Nfine * 24 + rows * slab), ("y_expand", rows * slab + Nfine * 24)):
+        if k in kernels:
+            n_launch = kernels[k]["launches"] / max(1, kernels["spread" if k == "y_expand" else "interp"]["launches"])
+            gbs = nbytes / (kernels[k]["avg_ms"] * n_launch / 1e3) / 1e9
+            binding[k] = {"roof": "hbm", "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    out["binding"] = binding
+    return out
+
+
+def run_recon(la, synth, work, graph: bool):
+    """Plain-form latency of the configuration's whole job (lam_mbir), and the graph form's
+    (second submission to a one-slot server, after one that creates plans and graphs)."""
+    def project(mp, scan, dp):
+        return la.forward(mp, la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp))
+
+    datasets, _ = synth.simulate(work, project)
+    prm = synth.recon_params(work)
+    tr0 = time.perf_counter()
+    rec, st = la.api._mbir(work.method, datasets, work.recon_dims, prm, stats=True)
+    tr = time.perf_counter() - tr0
+    out = {"method": f"MBIR{work.method}", "recon_s": round(tr, 4), "setup_s": round(st["setup_s"], 4),
+           "solve_s": round(st["solve_s"], 4), "normal_ops": st["normal_ops"], "cg_iters": st["cg_iters"],
+           "outer_iters": st["outer_iters"], "finite": bool(np.isfinite(rec).all())}
+    if work.method == 3:
+        out["s_per_iteration"] = round(st["solve_s"] / max(1, st["outer_iters"]), 5)
+        out["lipschitz"] = st["lipschitz"]
+    elif st["cg_iters"]:
+        out["cg_iters_per_s"] = round(st["cg_iters"] / st["solve_s"], 2)
+    if graph:
+        from lamino_amd.serve import Server
+
+        with Server(slots=1, mode="graph") as srv:
+            srv.wait(srv.submit(work.method, datasets, work.recon_dims, prm))
+            tg0 = time.perf_counter()
+            recg, info = srv.wait(srv.submit(work.method, datasets, work.recon_dims, prm))
+            tg = time.perf_counter() - tg0
+        sg = info["stats"]
+        out["graph"] = {"recon_s": round(tg, 4), "setup_s": round(sg["setup_s"], 4), "solve_s": round(sg["solve_s"], 4),
+                        "rel_diff_vs_plain": _rel(recg, rec)}
+        if sg["cg_iters"]:
+            out["graph"]["cg_iters_per_s"] = round(sg["cg_iters"] / sg["solve_s"], 2)
+    return out
+
+
 def _simulate_jobs(la, synth, specs):
     """Measured projections of every job (GPU forward projector, src/forward.cpp recipe)."""
     def project(mp, scan, dp):
-        pg = la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp)
-        return la.forward(mp, pg)
+        return la.forward(mp, la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp))
 
     jobs = []
     for sp in specs:
@@ -78,16 +303,10 @@ def _simulate_jobs(la, synth, specs):
     return jobs
 
 
-def _rel(a, b):
-    nb = float(np.linalg.norm(b))
-    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
-
-
 def run_batched(la, synth, njobs, slots):
-    """Reconstructions/hour of a mix of uneven jobs: the plain form (one job at a
-    time, lam_mbir) against the batched form (all jobs submitted at once to a
-    server: concurrent slots, hipGraph-replayed solver iterations, autotuned
-    spreader tiles).  Results of the two forms are compared job by job."""
+    """Reconstructions/hour of a mix of uneven jobs: the plain form (one job at a time,
+    lam_mbir) against the batched form (all jobs submitted at once to a server: concurrent
+    slots, hipGraph-replayed solver iterations, autotuned spreader tiles)."""
     from lamino_amd.serve import Server
 
     specs = synth.batched_specs(njobs)
@@ -105,10 +324,6 @@ def run_batched(la, synth, njobs, slots):
         t_batch = time.perf_counter() - t0
         st = srv.stats()
     diffs = [_rel(x, p) for (x, _), p in zip(res, plain)]
-    for (m, ds, rd, prm, _), (x, info) in zip(jobs, res):
-        log(f"  job MBIR{m} vol {rd} x {len(ds)}x{ds[0][0].shape}: slot {info['slot']} "
-            f"start {info['start_s']:.2f} end {info['end_s']:.2f} setup {info['stats']['setup_s']:.3f} "
-            f"solve {info['stats']['solve_s']:.3f}")
     out = {"jobs": len(jobs), "slots": slots, "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
            "recon_per_hour_plain": round(3600 * len(jobs) / t_plain, 1),
            "recon_per_hour_batched": round(3600 * len(jobs) / t_batch, 1),
@@ -121,8 +336,8 @@ def run_batched(la, synth, njobs, slots):
 
 
 def run_served(la, synth, njobs, slots):
-    """Continuous batching under Poisson arrivals: jobs are submitted at their
-    arrival times while others run; latency = arrival -> result on host."""
+    """Continuous batching under Poisson arrivals: jobs are submitted at their arrival
+    times while others run; latency = arrival -> result on host."""
     from lamino_amd.serve import Server
 
     specs = synth.served_specs(njobs)
@@ -141,16 +356,57 @@ def run_served(la, synth, njobs, slots):
         t_all = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array([info["latency_s"] for _, info in res])
-    nonfinite = [f"MBIR{m} vol {rd}" for (m, ds, rd, prm, _), (x, _) in zip(jobs, res) if not np.isfinite(x).all()]
+    finite = np.array([bool(np.isfinite(x).all()) for x, _ in res])
+    nonfinite = [f"MBIR{m} vol {rd}" for (m, ds, rd, prm, _), ok in zip(jobs, finite) if not ok]
     waits = np.array([info["start_s"] - info["submit_s"] for _, info in res])
+    arrival_span = max(j[4] for j in jobs)
+    thr = len(jobs) / t_all
+    rate = len(jobs) / arrival_span if arrival_span > 0 else float("inf")
     out = {"jobs": len(jobs), "slots": slots, "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
            "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_queue_wait_s": round(float(waits.mean()), 3),
-           "throughput_jobs_per_s": round(len(jobs) / t_all, 4), "makespan_s": round(t_all, 3),
-           "nonfinite_jobs": nonfinite, "peak_admitted_bytes": st["peak_in_use"],
+           "throughput_jobs_per_s": round(thr, 4), "arrival_rate_jobs_per_s": round(rate, 4),
+           "saturated": bool(thr < rate), "makespan_s": round(t_all, 3), "nonfinite_jobs": nonfinite,
+           "peak_admitted_bytes": st["peak_in_use"],
            "mix": "Poisson arrivals 4 jobs/s, volumes up to [51,511,511]x3, 2-4 gammas x 360 angles, "
-                  "split-Bregman or qGGMRF 50/50, seeded"}
+                  "split-Bregman (mu 10) or qGGMRF (mu = the normal operator's Lipschitz estimate) 50/50, seeded"}
+    if finite.any():
+        out["p50_latency_finite_s"] = round(float(np.percentile(lat[finite], 50)), 3)
+        out["p99_latency_finite_s"] = round(float(np.percentile(lat[finite], 99)), 3)
     log("served:", out)
     return out
+
+
+def cpu_baseline(pr, ps):
+    """The oracle's FP32 restatement of the reference path (ES width 7, sigma = 2 NUFFT per
+    component with per-call point setup, single-threaded detector FFTs), all host threads,
+    on a bounded sample: forward + adjoint of ps projections of the first dataset."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    oracle.set_threads(cores)
+    sc = pr.work.scans[0]
+    Ps = min(ps, pr.P)
+    log(f"CPU oracle baseline on {cores} threads: forward+adjoint of {Ps} projections")
+    t0 = time.perf_counter()
+    pc = oracle.forward(pr.m, sc.alpha[:Ps], sc.gamma[:Ps], pr.det, pr.det, eps=1.2e-6, prec=32)
+    oracle.adjoint(pc, pr.vol, sc.alpha[:Ps], sc.gamma[:Ps], eps=1.2e-6, prec=32)
+    tcpu = time.perf_counter() - t0
+    gp = pr.projs[0].download()[:Ps]
+    return {"value": round(Ps / tcpu, 3), "unit": "projections/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"forward+adjoint of {Ps} projections of the gamma=0 dataset at the padded shapes "
+                      f"{list(pr.vol)} / {pr.det}x{pr.det} (FP32, ES width 7), {tcpu:.2f} s",
+            "gpu_vs_cpu_forward_rel_l2": _rel(gp, pc)}
+
+
+WORKLOAD_KEY = {1: "plain_cg_89x21x89_det89", 2: "split_bregman_179x43x179_det305",
+                3: "precision_study_359x71x359_det615"}
+
+
+def describe(work, pr):
+    return (f"{work.name}; padded volume {list(pr.vol)}, detector {pr.det}x{pr.det}, {pr.G} x {pr.P} projections; "
+            "step = forward + adjoint of all datasets (3 components); point binning built once per grid")
 
 
 def main():
@@ -162,70 +418,18 @@ def main():
 
     import lamino_amd as la
     from lamino_amd import _lib, synth
-    from lamino_amd.api import _dims, _ptrs
 
-    work = synth.config(2)
-    vol, det = work.solver_shapes()           # (179, 43, 179), 305
-    G = len(work.scans)
-    P = len(work.scans[0].alpha)
-    log(f"workload: {work.name}; padded volume {vol}, detector {det}x{det}, {G} x {P} projections")
-
-    ctx = la.default_context()
-    lib = ctx.lib
-    eps = la.api.DEFAULT_EPS
-
-    # inputs: the padded vortex field and one projection stack per dataset, all resident on the device
-    m = np.zeros((3,) + vol, np.float32)
-    rd = work.recon_dims
-    o = [(v - n) // 2 for v, n in zip(vol, rd)]
-    m[:, o[0]:o[0] + rd[0], o[1]:o[1] + rd[1], o[2]:o[2] + rd[2]] = synth.vortex_field(rd, work.seed)
-    md = ctx.to_device(m)
-    grids = [la.PolarGrid.per_projection(s.alpha, s.gamma, det, det) for s in work.scans]
-    projs = [ctx.alloc((P, det, det), np.float32) for _ in range(G)]
-    outs = [ctx.alloc((3,) + vol, np.float32) for _ in range(G)]
-    dims = _dims(vol)
-
-    def step():
-        for g in range(G):
-            a, b, c = md.comps()
-            _lib.check(lib.lam_forward(ctx.h, grids[g].h, dims, eps, a, b, c, projs[g].ptr))
-            x, y, z = outs[g].comps()
-            _lib.check(lib.lam_adjoint(ctx.h, grids[g].h, dims, eps, projs[g].ptr, x, y, z))
-
-    log("warmup (includes per-grid point binning)")
-    for _ in range(max(args.warmup, 1)):
-        step()
-    ctx.sync()
-    # The timed region records HIP events around the spreader only (the dominant
-    # kernel, whose launch time the roofline needs): every timed scope adds two
-    # event records to the stream, and timing all ~28 launches of a step costs
-    # about 5 % of the step.
-    _lib.check(lib.lam_timing_enable(ctx.h, 1))
-    _lib.check(lib.lam_timing_only(ctx.h, b"spread"))
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.sync()
-    t1 = time.perf_counter()
-    spread_tot, spread_cnt = ctypes.c_double(), ctypes.c_int64()
-    _lib.check(lib.lam_timing_query(ctx.h, b"spread", ctypes.byref(spread_tot), ctypes.byref(spread_cnt)))
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = G * P / ((t1 - t0) / args.steps)
+    work = synth.config(args.config)
+    pr = Projectors(la, work)
+    log(f"headline workload: {describe(work, pr)}")
+    ms_per_step, spread_ms, spread_cnt = pr.timed(args.steps, args.warmup)
+    value = pr.G * pr.P / (ms_per_step / 1e3)
     log(f"fwd+adj: {ms_per_step:.3f} ms/step -> {value:.1f} projections/s")
+
     if args.profile:
-        # a few CG iterations of the Toeplitz normal operator + ramp preconditioner
-        T = la.ToeplitzOperator(grids, vol)
-        x0 = ctx.alloc((3,) + vol, np.float32)
-        x0.zero()
-        xd = ctx.alloc((3,) + vol, np.float32)
-        op = _lib.OP_FN(ctypes.cast(lib.lam_toep_op, ctypes.c_void_p).value)
-        it = ctypes.c_int64()
-        _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(outs[0].comps()), _ptrs(x0.comps()), 5, 0.0,
-                                    _ptrs(xd.comps()), ctypes.byref(it)))
-        ctx.sync()
-        log(f"ran {it.value} CG iterations")
-        # plain copy kernel of known size: the FETCH_SIZE / WRITE_SIZE calibration
+        cg = pr.cg(5)
+        log(f"CG: {cg}")
+        ctx, lib = pr.ctx, pr.ctx.lib
         nbytes = 1 << 30
         src, dst = ctx.alloc((nbytes // 4,), np.float32), ctx.alloc((nbytes // 4,), np.float32)
         src.zero()
@@ -235,178 +439,57 @@ def main():
         log(f"copied {nbytes} bytes device-to-device twice")
         return
 
-    # per-kernel breakdown: a separate diagnostic pass with every scope timed
-    diag_steps = min(args.steps, 5)
-    _lib.check(lib.lam_timing_only(ctx.h, None))
-    _lib.check(lib.lam_timing_enable(ctx.h, 1))
-    for _ in range(diag_steps):
-        step()
-    ctx.sync()
-    kernels = {}
-    for name in ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points",
-                 "spread_rows2d", "y_expand", "y_contract", "interp_rows2d"):
-        tot, cnt = ctypes.c_double(), ctypes.c_int64()
-        _lib.check(lib.lam_timing_query(ctx.h, name.encode(), ctypes.byref(tot), ctypes.byref(cnt)))
-        if cnt.value:
-            kernels[name] = {"total_ms": tot.value, "launches": cnt.value, "avg_ms": tot.value / cnt.value}
-    _lib.check(lib.lam_timing_enable(ctx.h, 0))
+    diag_steps = min(args.steps, 3)
+    kernels = pr.diagnostic(diag_steps)
     log("kernel time per step (ms, diagnostic pass):",
         {k: round(v["total_ms"] / diag_steps, 3) for k, v in kernels.items()})
+    roof = roofline(pr, kernels, spread_ms, spread_cnt, WORKLOAD_KEY[args.config])
+    log("roofline:", roof)
+    cg = pr.cg(args.cg_iters)
+    log("CG:", cg)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pr, args.cpu_projections or {1: 24, 2: 12, 3: 4}[args.config])
+        log("cpu:", cpu)
+    head = {"workload": describe(work, pr), "padded_volume": list(pr.vol), "detector": pr.det, "datasets": pr.G,
+            "projections": pr.P, "nufft_eps": la.api.DEFAULT_EPS, "point_binning": "cached per grid (built in warmup)",
+            "bin_build_ms": pr.bin_ms}
+    kernels_ms = {k: round(v["total_ms"] / diag_steps, 4) for k, v in kernels.items()}
+    pr.release()
 
-    # roofline of the dominant NUFFT kernel: algorithmic bytes per launch M*(12+24) + Nfine*24
-    # (coordinates + 3 complex components per point, fine grid touched once)
-    nf = [2 * n for n in vol]
-    from lamino_amd.synth import pad_n  # noqa: F401  (shape rules live there)
+    extra = {}
+    for ci in [int(c) for c in args.extra_configs.split(",") if c.strip()]:
+        if ci == args.config:
+            continue
+        w = synth.config(ci)
+        p2 = Projectors(la, w)
+        ms2, _, _ = p2.timed(max(5, args.steps), args.warmup)
+        extra[f"config{ci}"] = {"workload": describe(w, p2), "projections_per_s": round(p2.G * p2.P / (ms2 / 1e3), 2),
+                                "ms_per_step": round(ms2, 4), **p2.cg(args.cg_iters)}
+        log(f"config {ci}:", extra[f"config{ci}"])
+        p2.release()
 
-    def smooth_even(n):
-        while True:
-            r = n
-            for p_ in (2, 3, 5):
-                while r % p_ == 0:
-                    r //= p_
-            if r == 1 and n % 2 == 0:
-                return n
-            n += 1
-
-    nf = [smooth_even(max(2 * n, 12)) for n in vol]
-    Nfine = nf[0] * nf[1] * nf[2]
-    M = P * det * det
-    bytes_alg = M * 36 + Nfine * 24
-    dom = max(("interp", "spread"), key=lambda k: kernels.get(k, {"total_ms": 0})["total_ms"])
-    # the dominant kernel's launch time from the timed region (the diagnostic pass's
-    # figure only if the interpolator were to dominate)
-    avg_ms = spread_tot.value / spread_cnt.value if dom == "spread" and spread_cnt.value else kernels[dom]["avg_ms"]
-    achieved = bytes_alg / (avg_ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic_from_profiles(dom), "kernel": dom,
-                "bytes_per_launch": bytes_alg, "avg_launch_ms": round(avg_ms, 4)}
-    # The roofs that bind the NUFFT kernels.  These datasets share their tilt, so the
-    # projectors run row-factorised: per half-plane point the 2-D spreader
-    # read-modify-writes W^2 cells x 3 complex components (2 x 24 B each) in its LDS
-    # box and the 2-D interpolator reads them once (LDS roof; peak = 256 CUs x 128 B/clk
-    # x 2.4 GHz, an estimate from the CU's LDS width, not a published figure); the y
-    # passes stream the row slabs and the fine grid (HBM roof): y_contract reads the
-    # fine grid once and writes one (z, x) slab per row, y_expand reads the slabs
-    # (plus halo strips, not counted) and read-modify-writes the fine grid.
-    w = 6
-    half_points = P * ((det * det + 1) // 2)           # per launch: one dataset
-    rows = ((det * det + 1) // 2 - 1) // det + 1        # detector rows of the half set
-    slab = nf[0] * nf[2] * 24
-    lds_peak_tbs = 256 * 128 * 2.4e9 / 1e12
-    binding = {}
-    for k, per_pt in (("spread_rows2d", 2 * 24 * w ** 2), ("interp_rows2d", 24 * w ** 2)):
-        if k in kernels:
-            tbs = half_points * per_pt / (kernels[k]["avg_ms"] / 1e3) / 1e12
-            binding[k] = {"roof": "lds", "achieved_tbs": round(tbs, 2), "peak_tbs": round(lds_peak_tbs, 1),
-                          "frac": round(tbs / lds_peak_tbs, 3), "bytes_per_launch": half_points * per_pt}
-    for k, nbytes in (("y_contract", Nfine * 24 + rows * slab), ("y_expand", rows * slab + 2 * Nfine * 24)):
-        if k in kernels:
-            gbs = nbytes / (kernels[k]["avg_ms"] / 1e3) / 1e9
-            binding[k] = {"roof": "hbm", "achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK_GBS,
-                          "frac": round(gbs / HBM_PEAK_GBS, 3), "bytes_per_launch": nbytes}
-    roofline["binding"] = binding
-
-    # CG iterations/s: Toeplitz normal operator + ramp preconditioner at the same shape
-    log("building Toeplitz normal operator")
-    tb0 = time.perf_counter()
-    T = la.ToeplitzOperator(grids, vol)
-    ctx.sync()
-    toep_build_s = time.perf_counter() - tb0
-    b = outs[0]
-    xd = ctx.alloc((3,) + vol, np.float32)
-    x0 = ctx.alloc((3,) + vol, np.float32)
-    x0.zero()
-    op = _lib.OP_FN(ctypes.cast(lib.lam_toep_op, ctypes.c_void_p).value)
-    it = ctypes.c_int64()
-    _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), 2, 0.0,
-                                _ptrs(xd.comps()), ctypes.byref(it)))
-    ctx.sync()
-    tc0 = time.perf_counter()
-    _lib.check(lib.lam_cgsolver(ctx.h, dims, op, T.h, _ptrs(b.comps()), _ptrs(x0.comps()), args.cg_iters, 0.0,
-                                _ptrs(xd.comps()), ctypes.byref(it)))
-    ctx.sync()
-    cg_s = time.perf_counter() - tc0
-    cg_rate = it.value / cg_s
-    log(f"CG: {it.value} iterations in {cg_s:.3f} s -> {cg_rate:.1f} iters/s (Toeplitz build {toep_build_s:.3f} s)")
-
-    # plain-form reconstruction latency of the whole split-Bregman job
     recon = None
     if not args.no_recon:
-        log("simulating measured projections and running the split-Bregman job")
-
-        def project(mp, scan, dp):
-            pg = la.PolarGrid.per_projection(scan.alpha, scan.gamma, dp, dp)
-            return la.forward(mp, pg)
-
-        datasets, _ = synth.simulate(work, project)
-        prm = la.ReconParams(max_iters=work.params["max_iters"], inner_iters=work.params["inner_iters"],
-                             lam=work.params["lam"], mu=work.params["mu"], tol=0.0, xtol=0.0)
-        tr0 = time.perf_counter()
-        rec, st = la.MBIR2(datasets, rd, prm, stats=True)
-        tr = time.perf_counter() - tr0
-        recon = {"recon_s": round(tr, 4), "setup_s": round(st["setup_s"], 4), "solve_s": round(st["solve_s"], 4),
-                 "normal_ops": st["normal_ops"], "cg_iters": st["cg_iters"], "outer_iters": st["outer_iters"],
-                 "cg_iters_per_s": round(st["cg_iters"] / st["solve_s"], 2),
-                 "finite": bool(np.isfinite(rec).all())}
-        # the same job in graph form (device scalars, replayed hipGraph iterations); the first
-        # submission creates the worker's plans and captures the graphs, the second is timed
-        from lamino_amd.serve import Server
-
-        with Server(slots=1, mode="graph") as srv:
-            srv.wait(srv.submit(2, datasets, rd, prm))
-            tg0 = time.perf_counter()
-            recg, info = srv.wait(srv.submit(2, datasets, rd, prm))
-            tg = time.perf_counter() - tg0
-        sg = info["stats"]
-        recon["graph"] = {"recon_s": round(tg, 4), "setup_s": round(sg["setup_s"], 4),
-                          "solve_s": round(sg["solve_s"], 4), "cg_iters": sg["cg_iters"],
-                          "cg_iters_per_s": round(sg["cg_iters"] / sg["solve_s"], 2),
-                          "rel_diff_vs_plain": _rel(recg, rec)}
-        log("recon:", recon)
+        recon = {}
+        for ci in sorted({args.config, *[int(c) for c in args.extra_configs.split(",") if c.strip()]}):
+            w = synth.config(ci)
+            log(f"reconstruction job of configuration {ci} ({w.name})")
+            recon[f"config{ci}"] = run_recon(la, synth, w, graph=w.method != 3)
+            log(recon[f"config{ci}"])
 
     batched = None if args.no_batched else run_batched(la, synth, args.batched_jobs, args.slots)
     served = None if args.no_served else run_served(la, synth, args.served_jobs, args.slots)
 
-    # CPU baseline: the oracle's FP32 restatement of the reference path (w=7, sigma=2 ES NUFFT,
-    # per-component type-1/type-2, single-threaded detector FFTs), all host cores
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-        oracle.set_threads(cores)
-        sc = work.scans[0]
-        Ps = min(args.cpu_projections, P)
-        log(f"CPU oracle baseline on {cores} threads: forward+adjoint of {Ps} projections")
-        tcp0 = time.perf_counter()
-        pc = oracle.forward(m, sc.alpha[:Ps], sc.gamma[:Ps], det, det, eps=1.2e-6, prec=32)
-        oracle.adjoint(pc, vol, sc.alpha[:Ps], sc.gamma[:Ps], eps=1.2e-6, prec=32)
-        tcpu = time.perf_counter() - tcp0
-        gp = projs[0].download()[:Ps]
-        diff = float(np.linalg.norm(gp - pc) / np.linalg.norm(pc))
-        cpu = {"value": round(Ps / tcpu, 3), "unit": "projections/s", "cores": cores, "kind": "port",
-               "sample": f"forward+adjoint of {Ps} projections of the gamma=0 dataset at the padded shapes "
-                         f"(FP32, ES width 7), {tcpu:.2f} s",
-               "gpu_vs_cpu_forward_rel_l2": diff}
-        log("cpu:", cpu)
-
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "split-Bregman config: vol [32,128,128]x3 -> padded (179,43,179), det 128 -> 305x305, "
-                               "gammas (0,45) x 180 projections, tilt 35 deg; step = forward + adjoint of all datasets",
-                   "padded_volume": list(vol), "detector": det, "datasets": G, "projections": P,
-                   "nufft_eps": eps},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "cg_iters_per_s": round(cg_rate, 2),
-        "toeplitz_build_s": round(toep_build_s, 4),
-        "recon": recon,
-        "batched": batched,
-        "served": served,
-        "kernels_ms_per_step": {k: round(v["total_ms"] / diag_steps, 4) for k, v in kernels.items()},
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": head,
+        "roofline": roof, "cpu_baseline": cpu,
+        "cg_iters_per_s": cg["cg_iters_per_s"], "cg_ms_per_iter": cg["cg_ms_per_iter"],
+        "toeplitz_build_s": cg["toeplitz_build_s"],
+        "extra_configs": extra, "recon": recon, "batched": batched, "served": served,
+        "kernels_ms_per_step": kernels_ms,
     }
     print(json.dumps(line), flush=True)
 

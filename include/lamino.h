@@ -269,6 +269,38 @@ lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, in
 lam_status lam_server_wait(lam_server *server, int64_t job_id, lam_job_info *info);
 lam_status lam_server_query(lam_server *server, lam_server_stats *stats);
 
+/* ------------------------------------------------------- request ingest and output
+ * Host-side readers and writers of the reference's file formats (include/tiff.h,
+ * include/write_vti.h, include/config.h), for turning files into lam_dataset
+ * requests and results into files.  Paths are UTF-8; arrays are host memory. */
+
+/* Stack dims (pages, rows, cols) tiff::read returns (include/tiff.h:31-60): an
+ * even image width drops the last row and an even height the last column. */
+lam_status lam_tiff_info(const char *path, int64_t dims[3]);
+/* tiff::read (include/tiff.h:31-85): the first dims[1] rows and dims[2] columns
+ * of every page of 32-bit samples, taken as float; dims from lam_tiff_info. */
+lam_status lam_tiff_read(const char *path, float *out_host, const int64_t dims[3]);
+/* tiff::write (include/tiff.h:87-126): one float page per slice, one strip each. */
+lam_status lam_tiff_write(const char *path, const float *data_host, const int64_t dims[3]);
+/* tiff::write_vectors (include/tiff.h:128-174): 3 float samples per pixel, LZW. */
+lam_status lam_tiff_write_vectors(const char *path, const float *c0, const float *c1, const float *c2,
+                                  const int64_t dims[3]);
+/* tiff::write3 (include/tiff.h:176-188): files <name without .tif*>{0,1,2}.tiff. */
+lam_status lam_tiff_write3(const char *name, const float *c0, const float *c1, const float *c2,
+                           const int64_t dims[3]);
+/* vti::write_vectors (include/write_vti.h:36-111): <path with extension .vti>,
+ * appended raw Float64 vectors, origin at the volume centre; field_name NULL:
+ * "magnetization". */
+lam_status lam_vti_write_vectors(const char *path, const float *c0, const float *c1, const float *c2,
+                                 const int64_t dims[3], const char *field_name);
+/* read_angles_file<float> (include/config.h:61-83): out NULL queries the count. */
+lam_status lam_read_angles(const char *path, double *out, int64_t cap, int64_t *n);
+/* mask_infs_nans (include/mask.h:22-32) in place on n device floats. */
+lam_status lam_mask_infs_nans(lam_ctx *ctx, int64_t n, float *data);
+/* array::transpose(a, {1, 2, 0}) (include/array_ops.h:117-137), the output step of
+ * src/recon.cpp:76-78: in [n1][n2][n3] -> out [n2][n3][n1], device memory. */
+lam_status lam_transpose120(lam_ctx *ctx, const int64_t dims[3], const float *in, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,15 @@ _SIGS = {
                                          ctypes.POINTER(lam_recon_params), c_vp, c_i64p]),
     "lam_server_wait": (ctypes.c_int, [c_vp, c_i64, ctypes.POINTER(lam_job_info)]),
     "lam_server_query": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_server_stats)]),
+    "lam_tiff_info": (ctypes.c_int, [ctypes.c_char_p, c_i64p]),
+    "lam_tiff_read": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_i64p]),
+    "lam_tiff_write": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_i64p]),
+    "lam_tiff_write_vectors": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_vp, c_vp, c_i64p]),
+    "lam_tiff_write3": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_vp, c_vp, c_i64p]),
+    "lam_vti_write_vectors": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_vp, c_vp, c_i64p, ctypes.c_char_p]),
+    "lam_read_angles": (ctypes.c_int, [ctypes.c_char_p, c_dp, c_i64, c_i64p]),
+    "lam_mask_infs_nans": (ctypes.c_int, [c_vp, c_i64, c_fp]),
+    "lam_transpose120": (ctypes.c_int, [c_vp, c_i64p, c_fp, c_fp]),
 }
 
 _lib = None

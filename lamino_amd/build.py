@@ -30,7 +30,7 @@ COMMON_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["fft.hip", "nufft.hip", "ops.hip", "toeplitz.hip", "solvers.hip", "capi.hip", "serve.hip"]
+SOURCES = ["fft.hip", "nufft.hip", "ops.hip", "toeplitz.hip", "solvers.hip", "capi.hip", "serve.hip", "io.cpp"]
 
 
 def _compile(src: str) -> str:

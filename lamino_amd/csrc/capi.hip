@@ -622,6 +622,24 @@ lam_status lam_dot(lam_ctx *ctx, int64_t n, const float *a, const float *b, doub
     });
 }
 
+lam_status lam_mask_infs_nans(lam_ctx *ctx, int64_t n, float *data) {
+    return guard([&] {
+        need(ctx, "ctx"); need(data, "data");
+        LAM_REQUIRE(n >= 0, LAM_ERR_ARG, "n must be non-negative");
+        mask_infs_nans(ctx->impl, n, data);
+    });
+}
+
+lam_status lam_transpose120(lam_ctx *ctx, const int64_t dims[3], const float *in, float *out) {
+    return guard([&] {
+        need(ctx, "ctx"); need(in, "in"); need(out, "out");
+        long long d[3];
+        dims_of(dims, d);
+        LAM_REQUIRE(in != out, LAM_ERR_ARG, "transpose is out of place");
+        transpose120(ctx->impl, d, in, out);
+    });
+}
+
 lam_status lam_max(lam_ctx *ctx, int64_t n, const float *a, float *out) {
     return guard([&] {
         need(ctx, "ctx"); need(a, "a"); need(out, "out");

@@ -16,7 +16,6 @@
 #include "fastdiv.h"
 #include "nufft.h"
 
-#include <hipcub/device/device_radix_sort.hpp>
 
 namespace lam {
 
@@ -179,8 +178,7 @@ __global__ __launch_bounds__(256) void k_crop_deapod(const float2 *__restrict__ 
 template <int W, bool HALF>
 __global__ __launch_bounds__(256) void k_bin_keys(const Geo *__restrict__ geo, long long P, int R, int C, int nfz,
                                                   int nfy, int nfx, PointFrame fr, BinParams bp,
-                                                  unsigned *__restrict__ key, unsigned *__restrict__ val,
-                                                  unsigned *__restrict__ count) {
+                                                  unsigned *__restrict__ key, unsigned *__restrict__ count) {
     const long long S = (long long)R * C;
     const long long H = HALF ? (S + 1) / 2 : S;
     const long long total = P * H;
@@ -200,8 +198,61 @@ __global__ __launch_bounds__(256) void k_bin_keys(const Geo *__restrict__ geo, l
         const unsigned local =
             (unsigned)(((iz - bz * bp.T[0]) * bp.T[1] + (iy - by * bp.T[1])) * bp.T[2] + (ix - bxx * bp.T[2]));
         key[gid] = tile * tvol + local;
-        val[gid] = (unsigned)gid;
         atomicAdd(&count[tile], 1u);
+    }
+}
+
+// ------------------------------------------------------------------ bucket sort of the bins
+// The sort keys above are bucket * cells + cell, with buckets (a tile, or a
+// (row, tile) pair) counted into count[].  The sorted order is by bucket, then
+// by cell, then by point index.  Step 1 scatters every point into its bucket's
+// range (host-scanned offsets) at a position taken from an atomic cursor, so
+// the order inside a bucket is arbitrary there; step 2 gives every element of
+// a bucket its rank by (cell, point index), which is unique, so the result is
+// the same whatever order step 1 produced.
+__global__ __launch_bounds__(256) void k_bin_scatter(const unsigned *__restrict__ key, long long total, unsigned cells,
+                                                     const unsigned *__restrict__ offset, unsigned *__restrict__ cursor,
+                                                     unsigned long long *__restrict__ staged) {
+    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
+         gid += (long long)gridDim.x * blockDim.x) {
+        const unsigned k = key[gid];
+        const unsigned b = k / cells;   // keys reach 2^32: plain division (once per point, set-up only)
+        const unsigned cell = k - b * cells;
+        const unsigned pos = offset[b] + atomicAdd(&cursor[b], 1u);
+        staged[pos] = ((unsigned long long)cell << 32) | (unsigned long long)gid;
+    }
+}
+
+// One workgroup per bucket (blockIdx.x + first): rank = number of the bucket's
+// (cell, index) keys below one's own, counted against LDS tiles of the bucket.
+// Every thread runs the same loop trip counts (they depend on n only), so the
+// barriers are uniform.
+constexpr int kOrderTile = 1024;
+__global__ __launch_bounds__(256) void k_bin_order(const unsigned long long *__restrict__ staged,
+                                                   const unsigned *__restrict__ offset,
+                                                   const unsigned *__restrict__ count, unsigned first,
+                                                   unsigned *__restrict__ ids) {
+    __shared__ unsigned long long tile[kOrderTile];
+    const unsigned b = first + blockIdx.x;
+    const unsigned off = offset[b], n = count[b];
+    if (n == 0) return;
+    if (n == 1) {
+        if (threadIdx.x == 0) ids[off] = (unsigned)staged[off];
+        return;
+    }
+    for (unsigned i0 = 0; i0 < n; i0 += blockDim.x) {
+        const unsigned i = i0 + threadIdx.x;
+        const unsigned long long ki = i < n ? staged[off + i] : ~0ull;
+        unsigned rank = 0;
+        for (unsigned j0 = 0; j0 < n; j0 += kOrderTile) {
+            __syncthreads();
+            for (unsigned t = threadIdx.x; t < (unsigned)kOrderTile; t += blockDim.x)
+                tile[t] = j0 + t < n ? staged[off + j0 + t] : ~0ull;
+            __syncthreads();
+            const unsigned m = min((unsigned)kOrderTile, n - j0);
+            for (unsigned t = 0; t < m; ++t) rank += tile[t] < ki ? 1u : 0u;
+        }
+        if (i < n) ids[off + rank] = (unsigned)ki;
     }
 }
 
@@ -619,8 +670,7 @@ __device__ __forceinline__ int wrap_any(int i, int n) {
 template <int W>
 __global__ __launch_bounds__(256) void k_bin_keys_rows(const Geo *__restrict__ geo, long long P, int R, int C,
                                                        int nfz, int nfx, PointFrame fr, RowGeom rg,
-                                                       unsigned *__restrict__ key, unsigned *__restrict__ val,
-                                                       unsigned *__restrict__ count) {
+                                                       unsigned *__restrict__ key, unsigned *__restrict__ count) {
     const long long S = (long long)R * C, H = (S + 1) / 2, total = P * H;
     const unsigned tt = (unsigned)(rg.Tz * rg.Tx);
     for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < total;
@@ -635,7 +685,6 @@ __global__ __launch_bounds__(256) void k_bin_keys_rows(const Geo *__restrict__ g
         const int bz = iz / rg.Tz, bx = ix / rg.Tx;
         const unsigned jt = (unsigned)j * rg.ntiles + (unsigned)(bz * rg.ntx + bx);
         key[gid] = jt * tt + (unsigned)((iz - bz * rg.Tz) * rg.Tx + (ix - bx * rg.Tx));
-        val[gid] = (unsigned)gid;
         atomicAdd(&count[jt], 1u);
     }
 }
@@ -1248,6 +1297,40 @@ BinParams bin_params(Ctx &cx, const Grid &g, const NufftPlan &pl, int nc) {
     return make_bins(T, pl);
 }
 
+// Sorts the binned points: keys (bucket * cells + cell) of `total` points with
+// the per-bucket counts already on the host (hc) and on the device (count), into
+// point ids ordered by (bucket, cell, point index).  Returns the ids buffer.
+DevBuf bucket_sort(Ctx &cx, const DevBuf &key, long long total, unsigned cells, const std::vector<unsigned> &hc,
+                   const DevBuf &count) {
+    const size_t nb = hc.size();
+    std::vector<unsigned> ho(nb);
+    unsigned long long off = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        ho[b] = (unsigned)off;
+        off += hc[b];
+    }
+    LAM_REQUIRE((long long)off == total, LAM_ERR_STATE, "bin counts do not add up to the point count");
+    DevBuf offset(sizeof(unsigned) * nb), cursor(sizeof(unsigned) * nb);
+    DevBuf staged(sizeof(unsigned long long) * std::max<long long>(total, 1));
+    DevBuf ids(sizeof(unsigned) * std::max<long long>(total, 1));
+    upload_now(cx, offset.ptr, ho.data(), sizeof(unsigned) * nb);
+    LAM_HIP(hipMemsetAsync(cursor.ptr, 0, cursor.bytes, cx.stream));
+    const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, cx.stream, key.as<unsigned>(), total, cells,
+                       offset.as<unsigned>(), cursor.as<unsigned>(), staged.as<unsigned long long>());
+    check_launch();
+    constexpr size_t kMaxBlocks = (size_t)1 << 30;
+    for (size_t b0 = 0; b0 < nb; b0 += kMaxBlocks) {
+        const size_t nblk = std::min(kMaxBlocks, nb - b0);
+        hipLaunchKernelGGL(k_bin_order, dim3((unsigned)nblk), dim3(256), 0, cx.stream,
+                           staged.as<unsigned long long>(), offset.as<unsigned>(), count.as<unsigned>(),
+                           (unsigned)b0, ids.as<unsigned>());
+        check_launch();
+    }
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+    return ids;
+}
+
 template <class K>
 void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024) LAM_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -1263,10 +1346,7 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
     const long long ntiles = (long long)bp.nt[0] * bp.nt[1] * bp.nt[2];
     const long long nkeys = ntiles * bp.T[0] * bp.T[1] * bp.T[2];
     LAM_REQUIRE(nkeys <= (1ll << 32), LAM_ERR_ARG, "fine grid too large for 32-bit bin keys");
-    int key_bits = 1;
-    while (key_bits < 32 && (1ll << key_bits) < nkeys) ++key_bits;
-    DevBuf key0(sizeof(unsigned) * total), key1(sizeof(unsigned) * total);
-    DevBuf val0(sizeof(unsigned) * total), val1(sizeof(unsigned) * total);
+    DevBuf key0(sizeof(unsigned) * std::max<long long>(total, 1));
     DevBuf count(sizeof(unsigned) * ntiles);
     LAM_HIP(hipMemsetAsync(count.ptr, 0, count.bytes, cx.stream));
     const PointFrame fr = frame(g, pl);
@@ -1275,27 +1355,20 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
     if (half) {
         LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_keys<W, true>), dim3(grid), dim3(256), 0, cx.stream, geo, g.P,
                                                   (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
-                                                  key0.as<unsigned>(), val0.as<unsigned>(), count.as<unsigned>()));
+                                                  key0.as<unsigned>(), count.as<unsigned>()));
     } else {
         LAM_WIDTH_SWITCH(pl.w, hipLaunchKernelGGL((k_bin_keys<W, false>), dim3(grid), dim3(256), 0, cx.stream, geo,
                                                   g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[1], pl.nf[2], fr, bp,
-                                                  key0.as<unsigned>(), val0.as<unsigned>(), count.as<unsigned>()));
+                                                  key0.as<unsigned>(), count.as<unsigned>()));
     }
     check_launch();
-    // stable LSD radix sort of (key, point index): the order inside a key is the
-    // point order, so the binning and every sum over it are deterministic
-    hipcub::DoubleBuffer<unsigned> dk(key0.as<unsigned>(), key1.as<unsigned>());
-    hipcub::DoubleBuffer<unsigned> dv(val0.as<unsigned>(), val1.as<unsigned>());
-    size_t temp_bytes = 0;
-    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
-    DevBuf temp(std::max<size_t>(temp_bytes, 16));
-    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
-    auto sp = std::make_unique<SortedPoints>();
-    sp->npoints = total;
-    sp->ids = std::move(dv.Current() == val0.as<unsigned>() ? val0 : val1);
     std::vector<unsigned> hc(ntiles);
     LAM_HIP(hipMemcpyAsync(hc.data(), count.ptr, sizeof(unsigned) * ntiles, hipMemcpyDeviceToHost, cx.stream));
     LAM_HIP(hipStreamSynchronize(cx.stream));
+    // order: tile, then first cell inside the tile, then point index (deterministic)
+    auto sp = std::make_unique<SortedPoints>();
+    sp->npoints = total;
+    sp->ids = bucket_sort(cx, key0, total, (unsigned)(bp.T[0] * bp.T[1] * bp.T[2]), hc, count);
     std::vector<uint4> chunks;
     unsigned long long off = 0;
     for (long long t = 0; t < ntiles; ++t) {
@@ -1318,14 +1391,17 @@ SortedPoints &get_bins(Ctx &cx, Grid &g, NufftPlan &pl, bool half, const BinPara
 // ---- row-factorised path (host)
 constexpr int kTileRowsTarget = 16;     // preferred 2-D tile edge of the row path (measured: 16 < 20 < 24)
 
-// Slab memory of one row group: LAMINO_NUFFT_SLAB_BYTES, default 1 GiB.
+// Slab memory of one row group: LAMINO_NUFFT_SLAB_BYTES, default 8 GiB, which
+// holds the 308 rows x 22 MB of a (720, 144, 720) fine grid with a 615-point
+// detector in one group: one y pass that stores each plane once, instead of a
+// zero fill and a read-modify-write of the fine grid per group.
 size_t slab_budget() {
     const char *e = std::getenv("LAMINO_NUFFT_SLAB_BYTES");
     if (e && *e) {
         const long long v = std::atoll(e);
         if (v > 0) return (size_t)v;
     }
-    return (size_t)1 << 30;
+    return (size_t)8 << 30;
 }
 
 long long half_rows(const Grid &g) {
@@ -1431,28 +1507,20 @@ RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
     upload_now(cx, rp->iy.ptr, iyh.data(), sizeof(int) * iyh.size());
     rp->iy_host = iyh;
 
-    int key_bits = 1;
-    while (key_bits < 32 && (1ll << key_bits) < nkeys) ++key_bits;
     const long long ncount = (long long)rp->Jn * rg.ntiles;
-    DevBuf key0(sizeof(unsigned) * total), key1(sizeof(unsigned) * total);
-    DevBuf val0(sizeof(unsigned) * total), val1(sizeof(unsigned) * total);
+    DevBuf key0(sizeof(unsigned) * std::max<long long>(total, 1));
     rp->count.alloc(sizeof(unsigned) * ncount);
     LAM_HIP(hipMemsetAsync(rp->count.ptr, 0, rp->count.bytes, cx.stream));
     const unsigned grid = grid_for((size_t)total, 256, 1u << 20);
     LAM_WIDTH_SWITCH(W, hipLaunchKernelGGL((k_bin_keys_rows<W>), dim3(grid), dim3(256), 0, cx.stream,
                                            g.geo.as<Geo>(), g.P, (int)g.R, (int)g.C, pl.nf[0], pl.nf[2], fr, rg,
-                                           key0.as<unsigned>(), val0.as<unsigned>(), rp->count.as<unsigned>()));
+                                           key0.as<unsigned>(), rp->count.as<unsigned>()));
     check_launch();
-    hipcub::DoubleBuffer<unsigned> dk(key0.as<unsigned>(), key1.as<unsigned>());
-    hipcub::DoubleBuffer<unsigned> dv(val0.as<unsigned>(), val1.as<unsigned>());
-    size_t temp_bytes = 0;
-    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
-    DevBuf temp(std::max<size_t>(temp_bytes, 16));
-    LAM_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, temp_bytes, dk, dv, (int)total, 0, key_bits, cx.stream));
-    rp->ids = std::move(dv.Current() == val0.as<unsigned>() ? val0 : val1);
     std::vector<unsigned> hc(ncount);
     LAM_HIP(hipMemcpyAsync(hc.data(), rp->count.ptr, sizeof(unsigned) * ncount, hipMemcpyDeviceToHost, cx.stream));
     LAM_HIP(hipStreamSynchronize(cx.stream));
+    // order: row, tile, first (z, x) cell inside the tile, then point index (deterministic)
+    rp->ids = bucket_sort(cx, key0, total, (unsigned)(rg.Tz * rg.Tx), hc, rp->count);
     // one chunk per non-empty (row, tile): its workgroup alone writes that tile's
     // interior and halo record
     std::vector<uint4> chunks;

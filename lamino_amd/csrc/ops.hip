@@ -1,5 +1,8 @@
 // Fused elementwise, stencil, reduction and preconditioner kernels.
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "fft.h"
 #include "ops.h"
@@ -254,6 +257,49 @@ void reduce_terms(Ctx &cx, long long n, const RedTerm *terms, int nterms, double
     reduce_terms_dev(cx, n, terms, nterms, cx.scal.as<double>());
     LAM_HIP(hipMemcpyAsync(out_host, cx.scal.ptr, sizeof(double) * nterms, hipMemcpyDeviceToHost, cx.stream));
     LAM_HIP(hipStreamSynchronize(cx.stream));
+}
+
+// ---- request ingest / output helpers
+namespace {
+__global__ __launch_bounds__(256) void k_mask_nonfinite(float *a, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (!isfinite(a[i])) a[i] = 0.f;
+}
+
+// out[j][k][i] = in[i][j][k]: a 32 x 32 tile of the (i, jk) plane through LDS, so
+// that both the reads (along jk) and the writes (along i) are coalesced.
+__global__ __launch_bounds__(256) void k_transpose120(const float *__restrict__ in, float *__restrict__ out, int n1,
+                                                      long long m) {
+    __shared__ float t[32][33];
+    const long long c0 = (long long)blockIdx.x * 32;   // jk
+    const int r0 = (int)blockIdx.y * 32;               // i
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = r0 + r;
+        const long long jk = c0 + tx;
+        if (i < n1 && jk < m) t[r][tx] = in[(long long)i * m + jk];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long long jk = c0 + r;
+        const int i = r0 + tx;
+        if (i < n1 && jk < m) out[jk * n1 + i] = t[tx][r];
+    }
+}
+}  // namespace
+
+void mask_infs_nans(Ctx &cx, long long n, float *a) {
+    hipLaunchKernelGGL(k_mask_nonfinite, dim3(grid_for((size_t)n, 256, 1u << 16)), dim3(256), 0, cx.stream, a, n);
+    check_launch();
+}
+
+void transpose120(Ctx &cx, const long long d[3], const float *in, float *out) {
+    const long long m = d[1] * d[2];
+    const long long gx = (m + 31) / 32, gy = (d[0] + 31) / 32;
+    LAM_REQUIRE(gx < (1ll << 31) && gy <= 65535, LAM_ERR_ARG, "transpose: volume too large for one launch");
+    hipLaunchKernelGGL(k_transpose120, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, cx.stream, in, out, (int)d[0],
+                       m);
+    check_launch();
 }
 
 float max_value(Ctx &cx, long long n, const float *a) {
