@@ -265,7 +265,26 @@ lam_status lam_server_destroy(lam_server *server);
 lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, int ndatasets,
                              const int64_t recon_dims[3], const lam_recon_params *params, float *out3_host,
                              int64_t *job_id);
-/* Blocks until the job has finished; its error, if any, is the return value. */
+/* File output of a served job (src/recon.cpp:76-110): the result leaves the
+ * device as array::transpose(., {1, 2, 0}) when transpose120 is set (the recon
+ * CLI's order), goes to pinned memory, and a writer thread writes the files
+ * while the worker goes on with the next job; the job finishes when they are
+ * written.  Formats: tiff::write3 (<path without .tif*>{0,1,2}.tiff), VTI
+ * (<path>.vti), tiff::write_vectors (path itself). */
+enum { LAM_OUT_TIFF = 1, LAM_OUT_VTI = 2, LAM_OUT_TIFF_VECTORS = 4 };
+typedef struct {
+    const char *path;
+    int formats;          /* LAM_OUT_* flags */
+    int transpose120;
+} lam_output;
+/* lam_server_submit with file output; out3_host (may be NULL) also receives the
+ * written result.  The strings are copied. */
+lam_status lam_server_submit_output(lam_server *server, const lam_dataset *datasets, int ndatasets,
+                                    const int64_t recon_dims[3], const lam_recon_params *params,
+                                    const lam_output *output, float *out3_host, int64_t *job_id);
+/* Blocks until the job has finished; its error, if any, is the return value.
+ * A job is waited on once: its record is released then (a second wait on the
+ * same id is LAM_ERR_ARG, unknown job id). */
 lam_status lam_server_wait(lam_server *server, int64_t job_id, lam_job_info *info);
 lam_status lam_server_query(lam_server *server, lam_server_stats *stats);
 

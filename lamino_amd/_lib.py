@@ -51,6 +51,11 @@ class lam_server_stats(ctypes.Structure):
 
 
 LAM_SERVE_PLAIN, LAM_SERVE_ASYNC, LAM_SERVE_GRAPH = 0, 1, 2
+LAM_OUT_TIFF, LAM_OUT_VTI, LAM_OUT_TIFF_VECTORS = 1, 2, 4
+
+
+class lam_output(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_char_p), ("formats", ctypes.c_int), ("transpose120", ctypes.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -113,6 +118,9 @@ _SIGS = {
     "lam_server_destroy": (ctypes.c_int, [c_vp]),
     "lam_server_submit": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
                                          ctypes.POINTER(lam_recon_params), c_vp, c_i64p]),
+    "lam_server_submit_output": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_dataset), ctypes.c_int, c_i64p,
+                                                ctypes.POINTER(lam_recon_params), ctypes.POINTER(lam_output), c_vp,
+                                                c_i64p]),
     "lam_server_wait": (ctypes.c_int, [c_vp, c_i64, ctypes.POINTER(lam_job_info)]),
     "lam_server_query": (ctypes.c_int, [c_vp, ctypes.POINTER(lam_server_stats)]),
     "lam_tiff_info": (ctypes.c_int, [ctypes.c_char_p, c_i64p]),

@@ -70,7 +70,7 @@ using clk = std::chrono::steady_clock;
 
 // Runs one reconstruction job end to end on the context's stream.
 void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr, float *out3,
-              lam_recon_stats *stats, MbirMode mode) {
+              lam_recon_stats *stats, MbirMode mode, bool transpose120) {
     LAM_REQUIRE(nds >= 1, LAM_ERR_ARG, "MBIR needs at least one dataset");
     LAM_REQUIRE(pr.method >= 1 && pr.method <= 3, LAM_ERR_ARG, "unknown reconstruction method");
     const auto t0 = clk::now();
@@ -203,13 +203,19 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
     }
     // crop to the requested dims and download
     const long long m = rd[0] * rd[1] * rd[2];
-    DevBuf crop(sizeof(float) * m);
+    DevBuf crop(sizeof(float) * m), tr;
+    if (transpose120) tr.alloc(sizeof(float) * m);
     for (int c = 0; c < 3; ++c) {
         hipLaunchKernelGGL(k_crop3, dim3(vgrid(m)), dim3(256), 0, cx.stream, x.v().c[c], crop.as<float>(), (int)rd[0],
                            (int)rd[1], (int)rd[2], (int)od[1], (int)od[2], (int)((od[0] - rd[0]) / 2),
                            (int)((od[1] - rd[1]) / 2), (int)((od[2] - rd[2]) / 2));
         check_launch();
-        LAM_HIP(hipMemcpyAsync(out3 + c * m, crop.ptr, sizeof(float) * m, hipMemcpyDeviceToHost, cx.stream));
+        const float *res = crop.as<float>();
+        if (transpose120) {
+            transpose120_dev(cx, rd, crop.as<float>(), tr.as<float>());
+            res = tr.as<float>();
+        }
+        LAM_HIP(hipMemcpyAsync(out3 + c * m, res, sizeof(float) * m, hipMemcpyDeviceToHost, cx.stream));
     }
     LAM_HIP(hipStreamSynchronize(cx.stream));
     const auto t2 = clk::now();
@@ -251,8 +257,8 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
         raw = std::max(raw, 4 * P * (size_t)D.nrows * (size_t)D.ncols);
         padded = std::max(padded, 4 * M);
         spectra = std::max(spectra, 8 * M);
-        // sort keys (two buffers), the spare index buffer, radix-sort scratch, tile counts
-        bintmp = std::max(bintmp, 16 * half + 4 * tiles + ((size_t)16 << 20));
+        // sort keys, the bucket-scattered (cell, index) pairs, bucket offsets and cursors
+        bintmp = std::max(bintmp, 16 * half + 12 * tiles + ((size_t)16 << 20));
         // sorted ids, chunk list, geometry; twice: the autotuner holds two tile shapes at once
         bins += 2 * (4 * half + 16 * (half / 1024 + tiles)) + 128 * P;
         // the row-factorised projectors (grids sharing the tilt; counted for every grid)
@@ -267,7 +273,7 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
     if (pr.method == 1) solve += 48 * n + 8 * n;                   // r z p Ap + preconditioner
     if (pr.method == 2) solve += 24 * n + 72 * n + 48 * n + 8 * n; // xold rhs, d b, PCG
     if (pr.method == 3) solve += 36 * n;                           // xold y g
-    const size_t crop = 4 * (size_t)(rd[0] * rd[1] * rd[2]);
+    const size_t crop = 8 * (size_t)(rd[0] * rd[1] * rd[2]);      // cropped + transposed component
     const size_t slack = 64ull << 20;                              // plans, reductions, twiddles
     return 12 * n + raw + padded + spectra + bintmp + bins + rows_slab + fine + toep + solve + crop + slack;
 }
@@ -636,7 +642,7 @@ lam_status lam_transpose120(lam_ctx *ctx, const int64_t dims[3], const float *in
         long long d[3];
         dims_of(dims, d);
         LAM_REQUIRE(in != out, LAM_ERR_ARG, "transpose is out of place");
-        transpose120(ctx->impl, d, in, out);
+        transpose120_dev(ctx->impl, d, in, out);
     });
 }
 

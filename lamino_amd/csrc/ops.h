@@ -30,7 +30,7 @@ void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, do
 float max_value(Ctx &cx, long long n, const float *a);
 // non-finite values -> 0 (include/mask.h:22-32); out[j][k][i] = in[i][j][k]
 void mask_infs_nans(Ctx &cx, long long n, float *a);
-void transpose120(Ctx &cx, const long long d[3], const float *in, float *out);
+void transpose120_dev(Ctx &cx, const long long d[3], const float *in, float *out);
 
 // ---- stencils (src/finitediff.cpp:7-71, include/bregman.h:33-36)
 void grad_u(Ctx &cx, const long long d[3], const float *u, float *g0, float *g1, float *g2);

@@ -293,7 +293,7 @@ void mask_infs_nans(Ctx &cx, long long n, float *a) {
     check_launch();
 }
 
-void transpose120(Ctx &cx, const long long d[3], const float *in, float *out) {
+void transpose120_dev(Ctx &cx, const long long d[3], const float *in, float *out) {
     const long long m = d[1] * d[2];
     const long long gx = (m + 31) / 32, gy = (d[0] + 31) / 32;
     LAM_REQUIRE(gx < (1ll << 31) && gy <= 65535, LAM_ERR_ARG, "transpose: volume too large for one launch");

@@ -10,6 +10,7 @@
 // when it runs alone, since every buffer and reduction is the job's own.
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -33,9 +34,28 @@ struct Job {
     long long rd[3] = {0, 0, 0};
     lam_recon_params pr{};
     float *out = nullptr;
+    // file output (lam_server_submit_output): path, LAM_OUT_* formats, transposed result
+    std::string out_path;
+    int out_formats = 0;
+    bool out_transpose = false;
     std::string err;
     bool finished = false;
     lam_job_info info{};
+};
+
+// Pinned host buffers for results on their way to files: grow-only and reused, since
+// freeing pinned memory synchronises the device and would stall the other workers.
+struct Staging {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+};
+
+// A finished job's result in pinned memory, for the writer thread.
+struct WriteTask {
+    Job *job = nullptr;
+    Staging *buf = nullptr;
+    long long dims[3] = {0, 0, 0};   // of one component, as written
 };
 
 }  // namespace
@@ -57,13 +77,115 @@ struct Server {
     bool stopping = false;
     std::string start_error;
     std::vector<std::thread> workers;
+    // file output: pinned staging pool and the writer thread's queue
+    std::vector<std::unique_ptr<Staging>> staging;
+    std::deque<WriteTask> writes;
+    std::condition_variable cv_write, cv_staging;
+    bool writer_stop = false;
+    std::thread writer;
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
     // head of the queue fits the budget (callers hold mu)
     bool admissible() const { return !queue.empty() && in_use + queue.front()->info.footprint_est <= budget; }
 
     void worker(int slot);
+    void write_loop();
+    Staging *take_staging(size_t bytes);
+    void finish(Job *j, lam_status st, const std::string &err);
 };
+
+// A free pinned buffer of at least `bytes` (a new one when every free one is
+// smaller; waits while all are in use and the pool holds one per worker).
+Staging *Server::take_staging(size_t bytes) {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        Staging *best = nullptr;
+        for (auto &b : staging)
+            if (!b->busy && b->bytes >= bytes && (!best || b->bytes < best->bytes)) best = b.get();
+        if (best) {
+            best->busy = true;
+            return best;
+        }
+        size_t busy = 0;
+        for (auto &b : staging) busy += b->busy ? 1 : 0;
+        if (staging.size() < (size_t)(2 * slots) || busy < staging.size()) {
+            // grow: replace a free, too small buffer, or add one
+            Staging *slot = nullptr;
+            for (auto &b : staging)
+                if (!b->busy) { slot = b.get(); break; }
+            if (!slot) {
+                staging.push_back(std::make_unique<Staging>());
+                slot = staging.back().get();
+            }
+            slot->busy = true;
+            void *old = slot->ptr;
+            slot->ptr = nullptr;
+            slot->bytes = 0;
+            lk.unlock();
+            std::shared_lock<std::shared_mutex> cap(capture_mutex());
+            if (old) (void)hipHostFree(old);
+            void *p = nullptr;
+            const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+            lk.lock();
+            if (e != hipSuccess) {
+                slot->busy = false;
+                throw Error(LAM_ERR_NOMEM, "pinned staging for the result: " + std::string(hipGetErrorString(e)));
+            }
+            slot->ptr = p;
+            slot->bytes = bytes;
+            return slot;
+        }
+        cv_staging.wait(lk);
+    }
+}
+
+void Server::finish(Job *j, lam_status st, const std::string &err) {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        j->info.end_s = now();
+        j->info.status = st;
+        j->err = err;
+        j->finished = true;
+        if (st == LAM_OK) ++completed; else ++failed;
+    }
+    cv_done.notify_all();
+}
+
+// Writes finished jobs' files (TIFF and/or VTI) while the workers go on with the
+// next jobs, then copies the result to the caller's buffer if one was given.
+void Server::write_loop() {
+    for (;;) {
+        WriteTask t;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_write.wait(lk, [&] { return writer_stop || !writes.empty(); });
+            if (writes.empty()) return;
+            t = writes.front();
+            writes.pop_front();
+        }
+        Job *j = t.job;
+        const float *res = static_cast<const float *>(t.buf->ptr);
+        const long long m = t.dims[0] * t.dims[1] * t.dims[2];
+        const int64_t d[3] = {t.dims[0], t.dims[1], t.dims[2]};
+        lam_status st = LAM_OK;
+        std::string err;
+        const char *path = j->out_path.c_str();
+        if (st == LAM_OK && (j->out_formats & LAM_OUT_TIFF))
+            st = lam_tiff_write3(path, res, res + m, res + 2 * m, d);
+        if (st == LAM_OK && (j->out_formats & LAM_OUT_TIFF_VECTORS))
+            st = lam_tiff_write_vectors(path, res, res + m, res + 2 * m, d);
+        if (st == LAM_OK && (j->out_formats & LAM_OUT_VTI))
+            st = lam_vti_write_vectors(path, res, res + m, res + 2 * m, d, nullptr);
+        if (st != LAM_OK) err = lam_last_error();
+        if (st == LAM_OK && j->out) std::memcpy(j->out, res, sizeof(float) * 3 * (size_t)m);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            t.buf->busy = false;
+        }
+        cv_staging.notify_all();
+        finish(j, st, err);
+    }
+}
 
 void Server::worker(int slot) {
     Ctx cx;
@@ -98,8 +220,14 @@ void Server::worker(int slot) {
         tally->peak = tally->cur;
         lam_status st = LAM_OK;
         std::string err;
+        Staging *stage = nullptr;
         try {
-            mbir_run(cx, j->ds.data(), (int)j->ds.size(), j->rd, j->pr, j->out, &j->info.stats, mode);
+            float *dst = j->out;
+            if (j->out_formats) {
+                stage = take_staging(sizeof(float) * 3 * (size_t)(j->rd[0] * j->rd[1] * j->rd[2]));
+                dst = static_cast<float *>(stage->ptr);
+            }
+            mbir_run(cx, j->ds.data(), (int)j->ds.size(), j->rd, j->pr, dst, &j->info.stats, mode, j->out_transpose);
         } catch (const Error &e) {
             st = e.code;
             err = e.what();
@@ -119,21 +247,34 @@ void Server::worker(int slot) {
         cx.scal.release();
         (void)hipStreamSynchronize(cx.stream);
         const size_t peak = tally->peak - base;
+        const bool to_writer = st == LAM_OK && stage;
         {
             std::lock_guard<std::mutex> lk(mu);
             in_use -= j->info.footprint_est;
             --running;
-            j->info.end_s = now();
-            j->info.status = st;
             j->info.footprint_peak = peak;
-            j->err = err;
-            j->finished = true;
             j->proj.clear();
             j->proj.shrink_to_fit();
-            if (st == LAM_OK) ++completed; else ++failed;
+            if (stage && !to_writer) stage->busy = false;
+            if (to_writer) {
+                WriteTask t;
+                t.job = j;
+                t.buf = stage;
+                if (j->out_transpose) {
+                    t.dims[0] = j->rd[1]; t.dims[1] = j->rd[2]; t.dims[2] = j->rd[0];
+                } else {
+                    t.dims[0] = j->rd[0]; t.dims[1] = j->rd[1]; t.dims[2] = j->rd[2];
+                }
+                writes.push_back(t);
+            }
         }
-        cv_done.notify_all();
         cv_work.notify_all();
+        if (to_writer) {
+            cv_write.notify_all();
+        } else {
+            cv_staging.notify_all();
+            finish(j, st, err);
+        }
     }
 }
 
@@ -179,6 +320,7 @@ lam_status lam_server_create(int device, size_t hbm_budget, int slots, int mode,
             hbm_budget = freeb / 10 * 9;
         }
         S.budget = hbm_budget;
+        S.writer = std::thread([&S] { S.write_loop(); });
         for (int k = 0; k < slots; ++k) S.workers.emplace_back([&S, k] { S.worker(k); });
         {
             std::unique_lock<std::mutex> lk(S.mu);
@@ -191,6 +333,12 @@ lam_status lam_server_create(int device, size_t hbm_budget, int slots, int mode,
             }
             S.cv_work.notify_all();
             for (auto &t : S.workers) t.join();
+            {
+                std::lock_guard<std::mutex> lk(S.mu);
+                S.writer_stop = true;
+            }
+            S.cv_write.notify_all();
+            S.writer.join();
             throw Error(LAM_ERR_HIP, "server worker start failed: " + S.start_error);
         }
         *out = s.release();
@@ -207,24 +355,40 @@ lam_status lam_server_destroy(lam_server *server) {
         }
         S.cv_work.notify_all();
         for (auto &t : S.workers) t.join();
+        {
+            std::lock_guard<std::mutex> lk(S.mu);
+            S.writer_stop = true;   // the writer drains its queue first
+        }
+        S.cv_write.notify_all();
+        S.writer.join();
+        for (auto &b : S.staging)
+            if (b->ptr) (void)hipHostFree(b->ptr);
         delete server;
     });
 }
 
-lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, int nds, const int64_t recon_dims[3],
-                             const lam_recon_params *params, float *out3_host, int64_t *job_id) {
-    return guard([&] {
+namespace {
+void submit_job(lam_server *server, const lam_dataset *datasets, int nds, const int64_t recon_dims[3],
+                const lam_recon_params *params, float *out3_host, const lam_output *output, int64_t *job_id) {
         need(server, "server");
         need(datasets, "datasets");
         need(params, "params");
-        need(out3_host, "out3_host");
         need(job_id, "job_id");
+        LAM_REQUIRE(out3_host || output, LAM_ERR_ARG, "a job needs out3_host or an output");
         LAM_REQUIRE(nds >= 1, LAM_ERR_ARG, "MBIR needs at least one dataset");
         Server &S = server->impl;
         auto j = std::make_unique<Job>();
         dims_of(recon_dims, j->rd);
         j->pr = *params;
         j->out = out3_host;
+        if (output) {
+            need(output->path, "output.path");
+            LAM_REQUIRE(output->formats > 0 && (output->formats & ~(LAM_OUT_TIFF | LAM_OUT_VTI | LAM_OUT_TIFF_VECTORS)) == 0,
+                        LAM_ERR_ARG, "output formats must be a non-empty set of LAM_OUT_* flags");
+            j->out_path = output->path;
+            j->out_formats = output->formats;
+            j->out_transpose = output->transpose120 != 0;
+        }
         for (int g = 0; g < nds; ++g) {
             const lam_dataset &D = datasets[g];
             need(D.proj, "dataset.proj");
@@ -256,6 +420,23 @@ lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, in
             ++S.submitted;
         }
         S.cv_work.notify_all();
+}
+}  // namespace
+
+lam_status lam_server_submit(lam_server *server, const lam_dataset *datasets, int nds, const int64_t recon_dims[3],
+                             const lam_recon_params *params, float *out3_host, int64_t *job_id) {
+    return guard([&] {
+        need(out3_host, "out3_host");
+        submit_job(server, datasets, nds, recon_dims, params, out3_host, nullptr, job_id);
+    });
+}
+
+lam_status lam_server_submit_output(lam_server *server, const lam_dataset *datasets, int nds,
+                                    const int64_t recon_dims[3], const lam_recon_params *params,
+                                    const lam_output *output, float *out3_host, int64_t *job_id) {
+    return guard([&] {
+        need(output, "output");
+        submit_job(server, datasets, nds, recon_dims, params, out3_host, output, job_id);
     });
 }
 
@@ -270,7 +451,10 @@ lam_status lam_server_wait(lam_server *server, int64_t job_id, lam_job_info *inf
         Job *j = it->second.get();
         S.cv_done.wait(lk, [&] { return j->finished; });
         if (info) *info = j->info;
-        if (j->info.status != LAM_OK) throw Error(j->info.status, j->err);
+        const lam_status js = j->info.status;
+        const std::string jerr = j->err;
+        S.jobs.erase(it);   // a job is waited on once; its record goes with it
+        if (js != LAM_OK) throw Error(js, jerr);
     });
     return st;
 }

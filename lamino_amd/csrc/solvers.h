@@ -108,8 +108,10 @@ long long cg_graph_solve(Ctx &cx, const long long d[3], const OpFn &A, CVol3 b, 
 enum class MbirMode { Plain, Async, Graph };
 
 // One reconstruction job end to end on the context's stream (capi.hip).
+// transpose120: each component leaves as array::transpose(., {1, 2, 0}) of the
+// cropped volume ([rd1][rd2][rd0], the output order of src/recon.cpp:76-78).
 void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr, float *out3,
-              lam_recon_stats *stats, MbirMode mode);
+              lam_recon_stats *stats, MbirMode mode, bool transpose120 = false);
 // Upper bound of the device bytes mbir_run allocates for that job.
 size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], const lam_recon_params &pr);
 

@@ -176,7 +176,7 @@ def config(idx: int) -> Workload:
         return Workload("qGGMRF MBIR3: vol [51,256,256]x3, det 256, gammas (0,45,90) x 360 proj, tilt 30", 3,
                         recon_dims_internal([51, 256, 256]), oddify(256),
                         [lamino_scan(360, 30, g) for g in (0, 45, 90)], SEED_BASE + 3,
-                        params=dict(sigma=1000.0, p=1.2, mu=10.0, max_iters=50, inner_iters=3, tol=0.0, xtol=0.0))
+                        params=dict(sigma=1000.0, p=1.2, mu=0.0, max_iters=50, inner_iters=3, tol=0.0, xtol=0.0))
     raise ValueError(idx)
 
 
@@ -211,7 +211,10 @@ VOLUME_CLASSES = ([16, 64, 64], [32, 128, 128], [51, 256, 256])
 SERVED_CLASSES = (([16, 64, 64], 0.35), ([32, 128, 128], 0.35), ([51, 256, 256], 0.2), ([51, 511, 511], 0.1))
 SB_PARAMS = dict(lam=0.1, mu=10.0, max_iters=30, inner_iters=3, tol=0.0, xtol=0.0)
 CG_PARAMS = dict(max_iters=20, tol=0.0)
-QG_PARAMS = dict(sigma=1000.0, p=1.2, mu=10.0, max_iters=50, inner_iters=3, tol=0.0, xtol=0.0)
+# mu = 0: nagopt gets the normal operator's Lipschitz estimate as its Lipschitz constant.  The
+# reference passes mu (default 10) there (src/mbir3.cpp:123-125), a step of 0.01 that diverges
+# once 2-4 datasets of 360 projections put the operator's norm in the thousands.
+QG_PARAMS = dict(sigma=1000.0, p=1.2, mu=0.0, max_iters=50, inner_iters=3, tol=0.0, xtol=0.0)
 
 
 @dataclasses.dataclass

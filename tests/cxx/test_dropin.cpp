@@ -10,6 +10,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <stdexcept>
@@ -400,6 +401,16 @@ void test_double() {
     std::array<Array<double>, 3> f{Array<double>(d), Array<double>(d), Array<double>(d)};
     for (auto &c : f)
         for (auto &v : c) v = r.next();
+    // without the opt-in the double API refuses to compute in FP32
+    unsetenv("LAMINO_FP32_FOR_DOUBLE");
+    bool refused = false;
+    try {
+        (void)forward(f, pg, (double)s.gamma);
+    } catch (const std::invalid_argument &e) {
+        refused = std::strstr(e.what(), "LAMINO_FP32_FOR_DOUBLE") != nullptr;
+    }
+    report(N, "double refused without opt-in", refused ? 0.0 : 1.0, 0.5);
+    setenv("LAMINO_FP32_FOR_DOUBLE", "1", 1);
     Array<double> p = forward(f, pg, (double)s.gamma);
     long long dd[3];
     std::vector<double> fm, want(pg.size());

@@ -87,3 +87,38 @@ def test_recon_from_a_toml_request(ctx, orc, tmp_path):
     assert sorted(os.path.basename(w) for w in written) == ["recon.vti", "recon0.tiff", "recon1.tiff", "recon2.tiff"]
     for i in range(3):
         np.testing.assert_array_equal(ingest.read_tiff(str(tmp_path / "out" / f"recon{i}.tiff")), result[i])
+
+
+def test_served_requests_write_their_files_while_the_next_jobs_run(ctx, tmp_path):
+    """lam_server_submit_output: three requests on two slots; each job's result leaves the
+    device transposed {1, 2, 0}, its files are written by the server's writer thread, and
+    what is written equals the plain form's reconstruction in the same order."""
+    import lamino_amd as la
+    from lamino_amd import ingest
+    from lamino_amd.serve import Server
+
+    rng = np.random.default_rng(93)
+    jobs = []
+    for k, (rd, P, det) in enumerate([((11, 5, 13), 16, 13), ((9, 5, 9), 12, 11), ((13, 7, 11), 20, 15)]):
+        ds = []
+        for g in range(2):
+            al = np.full(P, math.radians(30.0))
+            ga = math.radians(45.0 * g) + 2 * np.pi * np.arange(P) / P
+            ds.append((rng.uniform(0.5, 1.5, (P, det, det)).astype(np.float32), al, ga))
+        jobs.append((rd, ds))
+    prm = la.ReconParams(max_iters=3, inner_iters=2, lam=0.1, mu=10.0, tol=0.0, xtol=0.0)
+    with Server(slots=2, mode="graph") as s:
+        ids = [s.submit_output(2, ds, rd, prm, str(tmp_path / f"job{k}" / "rec.tiff"), formats=("tiff", "vti"))
+               for k, (rd, ds) in enumerate(jobs)]
+        res = [s.wait(i) for i in ids]
+        with pytest.raises(Exception):
+            s.wait(ids[0])          # a job is waited on once
+    for k, ((rd, ds), (x, info)) in enumerate(zip(jobs, res)):
+        assert info["status"] == 0
+        plain = la.api._mbir(2, ds, rd, prm)
+        want = np.stack([np.transpose(plain[i], (1, 2, 0)) for i in range(3)])
+        assert x.shape == want.shape
+        assert rel(x, want) < 1e-5
+        for i in range(3):
+            np.testing.assert_array_equal(ingest.read_tiff(str(tmp_path / f"job{k}" / f"rec{i}.tiff")), x[i])
+        assert os.path.getsize(tmp_path / f"job{k}" / "rec.vti") > 3 * 8 * x[0].size
