@@ -130,6 +130,14 @@ lam_status lam_qggmrf(lam_ctx *ctx, const int64_t dims[3], float *g, const float
 /* array::dot / norm2 / max (include/array_ops.h:63-93), FP64 accumulation, fixed order */
 lam_status lam_dot(lam_ctx *ctx, int64_t n, const float *a, const float *b, double *out_host);
 lam_status lam_max(lam_ctx *ctx, int64_t n, const float *a, float *out_host);
+/* lam_dot without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels, launch
+ * shape and result as lam_dot; the FP64 partial sums live in a static per-device
+ * array of 1088 doubles (8.5 KiB, held until the process ends) instead of context
+ * memory.  n in [0, 2^40], else LAM_ERR_ARG.  Calls that overlap on one device
+ * share that array and race: serialise them (one stream, or wait in between). */
+lam_status lam_dot_noalloc(lam_ctx *ctx, int64_t n, const float *a, const float *b, double *out_host);
 
 /* ------------------------------------------------------------------- solvers */
 /* Operator callback: y = A(x) on device arrays, stream-ordered on the context stream. */

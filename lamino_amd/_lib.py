@@ -96,6 +96,7 @@ _SIGS = {
     "lam_qggmrf": (ctypes.c_int, [c_vp, c_i64p, c_fp, c_fp, ctypes.c_float, ctypes.c_float]),
     "lam_dot": (ctypes.c_int, [c_vp, c_i64, c_fp, c_fp, c_dp]),
     "lam_max": (ctypes.c_int, [c_vp, c_i64, c_fp, ctypes.POINTER(ctypes.c_float)]),
+    "lam_dot_noalloc": (ctypes.c_int, [c_vp, c_i64, c_fp, c_fp, c_dp]),
     "lam_cgsolver": (ctypes.c_int, [c_vp, c_i64p, OP_FN, c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_i64,
                                     ctypes.c_double, ctypes.POINTER(c_vp), c_i64p]),
     "lam_split_bregman": (ctypes.c_int, [c_vp, c_i64p, OP_FN, c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),

@@ -628,6 +628,13 @@ lam_status lam_dot(lam_ctx *ctx, int64_t n, const float *a, const float *b, doub
     });
 }
 
+lam_status lam_dot_noalloc(lam_ctx *ctx, int64_t n, const float *a, const float *b, double *out) {
+    return guard([&] {
+        need(ctx, "ctx"); need(a, "a"); need(b, "b"); need(out, "out");
+        *out = dot_noalloc(ctx->impl, n, a, b);
+    });
+}
+
 lam_status lam_mask_infs_nans(lam_ctx *ctx, int64_t n, float *data) {
     return guard([&] {
         need(ctx, "ctx"); need(data, "data");

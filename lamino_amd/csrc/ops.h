@@ -28,9 +28,13 @@ constexpr size_t kRedPartials = (size_t)kRedBlocks * kMaxRedTerms;
 void reduce_terms_dev(Ctx &cx, long long n, const RedTerm *terms, int nterms, double *out_dev,
                       double *partials = nullptr);
 float max_value(Ctx &cx, long long n, const float *a);
+// sum a*b in FP64 like reduce_terms, with its partial sums in a static device
+// array instead of context memory: no device allocation (n <= kNoAllocMaxElems)
+constexpr long long kNoAllocMaxElems = 1ll << 40;
 // non-finite values -> 0 (include/mask.h:22-32); out[j][k][i] = in[i][j][k]
 void mask_infs_nans(Ctx &cx, long long n, float *a);
 void transpose120_dev(Ctx &cx, const long long d[3], const float *in, float *out);
+double dot_noalloc(Ctx &cx, long long n, const float *a, const float *b);
 
 // ---- stencils (src/finitediff.cpp:7-71, include/bregman.h:33-36)
 void grad_u(Ctx &cx, const long long d[3], const float *u, float *g0, float *g1, float *g2);

@@ -302,6 +302,47 @@ void transpose120_dev(Ctx &cx, const long long d[3], const float *in, float *out
     check_launch();
 }
 
+// ---- the no-allocation dot product (lam_dot_noalloc)
+// Its scratch is this file-scope device array: the 1024 FP64 partial sums of the
+// fixed reduction grid and the result.  Its size does not depend on n.  Each
+// device holds its own copy at its own address until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for the
+// device (the array's name is a host-side stand-in, not a device address).
+namespace {
+constexpr size_t kNoAllocScratchDoubles = (size_t)kRedBlocks + 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;   // device -> address of that device's scratch
+}  // namespace
+
+// Looked up once per device, with the device current; later calls reuse it.
+double *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    LAM_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    LAM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    LAM_REQUIRE(p != nullptr, LAM_ERR_HIP, "no device address for the no-allocation scratch");
+    g_scratch_ptr.emplace(device, static_cast<double *>(p));
+    return static_cast<double *>(p);
+}
+
+double dot_noalloc(Ctx &cx, long long n, const float *a, const float *b) {
+    LAM_REQUIRE(n >= 0 && n <= kNoAllocMaxElems, LAM_ERR_ARG,
+                "lam_dot_noalloc: n = " + std::to_string(n) + " is outside [0, 2^40]");
+    double *ws = noalloc_scratch(cx.device);
+    const RedTerm t{RED_DOT, a, b};
+    // the same kernels and launch shape as lam_dot: partials in ws[0, kRedBlocks), the
+    // sum in ws[kRedBlocks]; every slot read is written first, so nothing needs zeroing
+    reduce_terms_dev(cx, n, &t, 1, ws + kRedBlocks, ws);
+    double out = 0.0;
+    LAM_HIP(hipMemcpyAsync(&out, ws + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, cx.stream));
+    LAM_HIP(hipStreamSynchronize(cx.stream));
+    return out;
+}
+
 float max_value(Ctx &cx, long long n, const float *a) {
     const int nb = 256;
     cx.red.ensure(sizeof(double) * kRedBlocks * kMaxRedTerms);
