@@ -32,7 +32,7 @@ sys.path.insert(0, HERE)
 METRIC = "fwd+adj NUFFT projections/s, CG iters/s vs plain form; p50/p99 recon latency"
 HBM_PEAK_GBS = 8000.0
 KERNEL_SCOPES = ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points", "spread_rows2d",
-                 "y_expand", "y_contract", "interp_rows2d", "toep_mul")
+                 "y_expand", "y_contract", "interp_rows2d", "toep_mul", "toep_xfused")
 
 
 def parse():
@@ -94,6 +94,19 @@ def smooth_even(n):
         if r == 1 and n % 2 == 0:
             return n
         n += 1
+
+
+def toep_len(n):
+    """The Toeplitz L-grid length of an axis of n voxels: smallest 2,3,5-smooth L >= 2n - 1."""
+    L = max(1, 2 * n - 1)
+    while True:
+        r = L
+        for p_ in (2, 3, 5):
+            while r % p_ == 0:
+                r //= p_
+        if r == 1:
+            return L
+        L += 1
 
 
 def _rel(a, b):
@@ -209,8 +222,22 @@ class Projectors:
                                     _ptrs(xd.comps()), ctypes.byref(it)))
         ctx.sync()
         cg_s = time.perf_counter() - tc0
-        return {"cg_iters_per_s": round(it.value / cg_s, 2), "cg_ms_per_iter": round(1e3 * cg_s / it.value, 4),
-                "toeplitz_build_s": round(build_s, 4)}
+        ms = 1e3 * cg_s / it.value
+        # algorithmic bytes of one PCG iteration (3 components): the Toeplitz normal
+        # operator on the half spectrum (volume in, 3 transforms forward and back,
+        # 6 kernel spectra, the multiply's read and write, volume out), the ramp
+        # preconditioner (volume in and out) and the fused vector step (x, p, r, Ap
+        # read; x, r written)
+        nv = int(np.prod(self.vol))
+        ng = int(np.prod([toep_len(n) for n in self.vol]))
+        op = 12 * nv + 3 * 2 * 16 * (ng // 2) + 6 * 4 * (ng // 2) + 3 * 8 * (ng // 2) * 2 + 12 * nv
+        it_bytes = op + 2 * 12 * nv + (4 * 12 + 2 * 12) * nv
+        gbs = it_bytes / (ms / 1e3) / 1e9
+        return {"cg_iters_per_s": round(it.value / cg_s, 2), "cg_ms_per_iter": round(ms, 4),
+                "toeplitz_build_s": round(build_s, 4),
+                "cg_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_iteration": it_bytes,
+                                "l_grid": [toep_len(n) for n in self.vol]}}
 
     def release(self):
         self.grids = self.projs = self.outs = self.md = None
@@ -487,7 +514,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": head,
         "roofline": roof, "cpu_baseline": cpu,
         "cg_iters_per_s": cg["cg_iters_per_s"], "cg_ms_per_iter": cg["cg_ms_per_iter"],
-        "toeplitz_build_s": cg["toeplitz_build_s"],
+        "cg_roofline": cg["cg_roofline"], "toeplitz_build_s": cg["toeplitz_build_s"],
         "extra_configs": extra, "recon": recon, "batched": batched, "served": served,
         "kernels_ms_per_step": kernels_ms,
     }

@@ -322,6 +322,20 @@ lam_status lam_vti_write_vectors(const char *path, const float *c0, const float 
                                  const int64_t dims[3], const char *field_name);
 /* read_angles_file<float> (include/config.h:61-83): out NULL queries the count. */
 lam_status lam_read_angles(const char *path, double *out, int64_t cap, int64_t *n);
+/* The forward-simulation CLI's computation (src/forward.cpp:123-183): the three
+ * components as read from TIFF ([d0][d1][d2] host arrays) are transposed
+ * {2, 0, 1}, padded by sqrt(2) - 1 (pad3d), projected on
+ * PolarGrid(angles, padded(rows), padded(cols), gamma), and the projections are
+ * cropped back to rows x cols: proj_host [nangles][rows][cols].  eps <= 0: 1e-5. */
+lam_status lam_forward_simulate(lam_ctx *ctx, const float *m0, const float *m1, const float *m2,
+                                const int64_t dims[3], const double *angles, int64_t nangles, double gamma,
+                                int64_t rows, int64_t cols, double eps, float *proj_host);
+/* opt::demag (include/demag.h:13-79): the demagnetising field
+ * H = F^-1[-k (k . F m) / |k|^2] of a device volume (DC zeroed), k in numpy's
+ * fftfreq order on every axis (on an even last axis the reference's r2c
+ * transform takes +n/2 for the Nyquist plane; that plane is where they differ). */
+lam_status lam_demag(lam_ctx *ctx, const int64_t dims[3], const float *m0, const float *m1, const float *m2,
+                     float *h0, float *h1, float *h2);
 /* mask_infs_nans (include/mask.h:22-32) in place on n device floats. */
 lam_status lam_mask_infs_nans(lam_ctx *ctx, int64_t n, float *data);
 /* array::transpose(a, {1, 2, 0}) (include/array_ops.h:117-137), the output step of

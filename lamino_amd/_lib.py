@@ -132,6 +132,9 @@ _SIGS = {
     "lam_vti_write_vectors": (ctypes.c_int, [ctypes.c_char_p, c_vp, c_vp, c_vp, c_i64p, ctypes.c_char_p]),
     "lam_read_angles": (ctypes.c_int, [ctypes.c_char_p, c_dp, c_i64, c_i64p]),
     "lam_mask_infs_nans": (ctypes.c_int, [c_vp, c_i64, c_fp]),
+    "lam_forward_simulate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64p, c_dp, c_i64, ctypes.c_double, c_i64,
+                                            c_i64, ctypes.c_double, c_vp]),
+    "lam_demag": (ctypes.c_int, [c_vp, c_i64p, c_fp, c_fp, c_fp, c_fp, c_fp, c_fp]),
     "lam_transpose120": (ctypes.c_int, [c_vp, c_i64p, c_fp, c_fp]),
 }
 

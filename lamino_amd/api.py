@@ -404,6 +404,17 @@ def dot(a: np.ndarray, b: np.ndarray, ctx: Context | None = None) -> float:
     return out.value
 
 
+def demag(m, ctx: Context | None = None) -> np.ndarray:
+    """opt::demag (include/demag.h:13-79): H = F^-1[-k (k . F m) / |k|^2] of the three
+    components m (3, n1, n2, n3), DC zeroed, k in numpy's fftfreq order on every axis."""
+    ctx = ctx or default_context()
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    md = ctx.to_device(m)
+    hd = ctx.alloc(m.shape, np.float32)
+    check(ctx.lib.lam_demag(ctx.h, _dims(m.shape[1:]), *md.comps(), *hd.comps()))
+    return hd.download()
+
+
 # ---------------------------------------------------------------- solvers
 
 

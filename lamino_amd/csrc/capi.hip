@@ -6,6 +6,7 @@
 #include <cstring>
 #include <limits>
 #include <new>
+#include <vector>
 
 #include "capi_util.h"
 #include "fft.h"
@@ -632,6 +633,64 @@ lam_status lam_dot_noalloc(lam_ctx *ctx, int64_t n, const float *a, const float 
     return guard([&] {
         need(ctx, "ctx"); need(a, "a"); need(b, "b"); need(out, "out");
         *out = dot_noalloc(ctx->impl, n, a, b);
+    });
+}
+
+// src/forward.cpp:123-183: transpose {2, 0, 1}, pad3d by sqrt(2) - 1 (float factor,
+// include/padding.h:168-204), forward projection on PolarGrid(angles, padded(rows),
+// padded(cols), gamma), crop2d back to rows x cols.
+lam_status lam_forward_simulate(lam_ctx *ctx, const float *m0, const float *m1, const float *m2,
+                                const int64_t dims[3], const double *angles, int64_t nangles, double gamma,
+                                int64_t rows, int64_t cols, double eps, float *proj_host) {
+    return guard([&] {
+        need(ctx, "ctx"); need(m0, "m0"); need(m1, "m1"); need(m2, "m2"); need(angles, "angles");
+        need(proj_host, "proj_host");
+        long long d[3];
+        dims_of(dims, d);
+        LAM_REQUIRE(nangles >= 1 && rows >= 1 && cols >= 1, LAM_ERR_ARG, "invalid projection shape");
+        Ctx &cx = ctx->impl;
+        constexpr double kPadding = 1.41421356237;
+        const float f = (float)(kPadding - 1);
+        const long long td[3] = {d[2], d[0], d[1]};   // transposed {2, 0, 1}
+        long long pd[3];
+        for (int a = 0; a < 3; ++a) pd[a] = td[a] + 2 * ((long long)((float)td[a] * f) / 2);
+        auto padded = [&](long long n) { return n + 2 * (long long)((double)n * (kPadding - 1) / 2); };
+        const long long R2 = padded(rows), C2 = padded(cols);
+        const long long n = d[0] * d[1] * d[2], np = pd[0] * pd[1] * pd[2];
+        DevBuf raw(sizeof(float) * n), vol(sizeof(float) * 3 * np);
+        LAM_HIP(hipMemsetAsync(vol.ptr, 0, vol.bytes, cx.stream));
+        const float *mh[3] = {m0, m1, m2};
+        float *mv[3];
+        for (int c = 0; c < 3; ++c) {
+            mv[c] = vol.as<float>() + c * np;
+            LAM_HIP(hipMemcpyAsync(raw.ptr, mh[c], sizeof(float) * n, hipMemcpyHostToDevice, cx.stream));
+            transpose201_pad(cx, d, raw.as<float>(), pd, mv[c]);
+        }
+        raw.release();
+        std::vector<double> gam((size_t)nangles, gamma);
+        auto grid = make_grid(cx, angles, gam.data(), nangles, R2, C2);
+        DevBuf proj(sizeof(float) * nangles * R2 * C2), out(sizeof(float) * nangles * rows * cols);
+        const float *mc[3] = {mv[0], mv[1], mv[2]};
+        forward(cx, *grid, pd, eps > 0 ? eps : 1e-5, mc, proj.as<float>());
+        crop2(cx, proj.as<float>(), nangles, (int)R2, (int)C2, out.as<float>(), (int)rows, (int)cols);
+        LAM_HIP(hipMemcpyAsync(proj_host, out.ptr, out.bytes, hipMemcpyDeviceToHost, cx.stream));
+        LAM_HIP(hipStreamSynchronize(cx.stream));
+    });
+}
+
+lam_status lam_demag(lam_ctx *ctx, const int64_t dims[3], const float *m0, const float *m1, const float *m2,
+                     float *h0, float *h1, float *h2) {
+    return guard([&] {
+        need(ctx, "ctx"); need(m0, "m0"); need(m1, "m1"); need(m2, "m2"); need(h0, "h0"); need(h1, "h1");
+        need(h2, "h2");
+        long long d[3];
+        dims_of(dims, d);
+        Ctx &cx = ctx->impl;
+        DevBuf work(sizeof(float2) * 3 * (size_t)(d[0] * d[1] * d[2]));
+        const float *m[3] = {m0, m1, m2};
+        float *h[3] = {h0, h1, h2};
+        demag(cx, d, m, h, work.as<float2>());
+        LAM_HIP(hipStreamSynchronize(cx.stream));
     });
 }
 

@@ -91,6 +91,14 @@ void det_fft2(Ctx &cx, const void *in, bool in_real, void *out, bool out_real, i
 void det_fft2_real_pairs(Ctx &cx, const float *proj, float2 *spec, int P, int R, int C, int sign);
 void det_ifft2_real_pairs(Ctx &cx, float2 *spec, float *proj, int P, int R, int C, int sign, float scale);
 
+// The Toeplitz normal operator's spectral step along the contiguous axis of the
+// L-grid: spectrum rows (z, y) of za/zb (transformed along z and y already,
+// nonzero at x < nvx) are transformed along x, multiplied by the 3x3 kernel
+// spectrum K ([L1][L2][L3][3], see toeplitz.h) and transformed back; wa/wb get
+// x < nvx of every row.  L3 must be 2,3,5-smooth.
+void toep_spectral_x(Ctx &cx, const float2 *za, const float2 *zb, const float2 *K, float2 *wa, float2 *wb, int L1,
+                     int L2, int L3, int nvx);
+
 // Full in-place 3-D complex FFT of [A][B][Cn][nc] (nc interleaved components).
 void fft3d(Ctx &cx, float2 *data, int A, int B, int Cn, int nc, int sign);
 

@@ -30,6 +30,8 @@ constexpr int kMaxLen = 3840;
 // this tile size and kLoadBatch, instead of 95.
 constexpr int kFftWaves = 6;
 constexpr int kMaxLines = 128;
+// Longest 4 lines of the fused Toeplitz spectral step (lines up to 1536 points).
+constexpr int kFusedMaxLen = 6144;
 
 // Elements per thread whose global loads (load phase) or LDS reads (store
 // phase) are issued together (8 measured the same and needs more registers).
@@ -870,6 +872,174 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
             hipLaunchKernelGGL((k_fft_axis<1, kThreads, kMaxLen, 1>), grid, dim3(kThreads), lds, cx.stream, P);
         else
             hipLaunchKernelGGL((k_fft_axis<0, kThreads, kMaxLen, 1>), grid, dim3(kThreads), lds, cx.stream, P);
+    }
+    check_launch();
+}
+
+// ------------------------------------------------------------------ Toeplitz spectral step
+// The stages of a 2,3,5-smooth plan, for kernels that run a whole transform on
+// LDS-resident lines between other work.
+struct StageList {
+    int nst = 0;
+    int rad[24] = {}, len[24] = {}, ns[24] = {}, sign[24] = {}, tw[24] = {};
+    const float2 *twp = nullptr;
+    FastDiv fNbf[24], fNs[24];
+};
+
+namespace {
+
+StageList make_stage_list(const FftPlan &plan) {
+    LAM_REQUIRE(!plan.blue && !plan.rader, LAM_ERR_STATE, "fused transform needs a 2,3,5-smooth length");
+    int pw;
+    const auto st = stage_list(plan, pw);
+    StageList S;
+    S.nst = (int)st.size();
+    S.twp = plan.tw.as<float2>();
+    for (int i = 0; i < S.nst; ++i) {
+        S.rad[i] = st[i].rad;
+        S.len[i] = st[i].len;
+        S.ns[i] = st[i].ns;
+        S.sign[i] = st[i].sign;
+        S.tw[i] = plan.tw_off[i];
+        S.fNbf[i] = FastDiv((unsigned)(st[i].len / st[i].rad));
+        S.fNs[i] = FastDiv((unsigned)st[i].ns);
+    }
+    return S;
+}
+
+template <int NT, int MAXLEN>
+__device__ __forceinline__ void run_stages(float2 *buf, const KParams &P, const SubLines &g, const StageList &S) {
+    for (int s = 0; s < S.nst; ++s) {
+        const float2 *tw = S.twp + S.tw[s];
+#define LAM_FSTAGE(R)                                                                                            \
+    case R:                                                                                                      \
+        stage<R, 0, NT, MAXLEN>(buf, P, g, false, S.len[s], S.ns[s], S.sign[s], tw, S.fNbf[s], S.fNs[s]);       \
+        break;
+        switch (S.rad[s]) {
+            LAM_FSTAGE(2) LAM_FSTAGE(3) LAM_FSTAGE(4) LAM_FSTAGE(5) LAM_FSTAGE(6) LAM_FSTAGE(8) LAM_FSTAGE(9)
+            LAM_FSTAGE(10) LAM_FSTAGE(12)
+            default: break;
+        }
+#undef LAM_FSTAGE
+    }
+}
+
+// X0, X1 from the packed spectrum Z = FFT(x0 + i x1) at f (a) and -f (b).
+__device__ __forceinline__ void unpack_pair(float2 a, float2 b, float2 &X0, float2 &X1) {
+    X0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    const float2 t = make_float2(a.x - b.x, a.y + b.y);
+    X1 = make_float2(0.5f * t.y, -0.5f * t.x);
+}
+
+// Y = K X with the real symmetric 3 x 3 spectrum (K00,K01),(K02,K11),(K12,K22), packed
+// back as (Y0 + i Y1, Y2) for the inverse transform's real-pair output.
+__device__ __forceinline__ void toep_block(float2 k0, float2 k1, float2 k2, float2 X0, float2 X1, float2 X2,
+                                           float2 &oa, float2 &ob) {
+    float2 Y0, Y1, Y2;
+    Y0.x = k0.x * X0.x + k0.y * X1.x + k1.x * X2.x;
+    Y0.y = k0.x * X0.y + k0.y * X1.y + k1.x * X2.y;
+    Y1.x = k0.y * X0.x + k1.y * X1.x + k2.x * X2.x;
+    Y1.y = k0.y * X0.y + k1.y * X1.y + k2.x * X2.y;
+    Y2.x = k1.x * X0.x + k2.x * X1.x + k2.y * X2.x;
+    Y2.y = k1.x * X0.y + k2.x * X1.y + k2.y * X2.y;
+    oa = make_float2(Y0.x - Y1.y, Y0.y + Y1.x);
+    ob = Y2;
+}
+
+// The normal operator's spectral step along the contiguous axis, fused: for the
+// spectrum row r = (z, y) (already transformed along z and y) and its mirror row
+// (-z, -y), the workgroup loads the four lines za[r], zb[r], za[mr], zb[mr]
+// (positions x < nvx; the rest of each line is the zero padding), transforms them
+// along x, multiplies by the kernel spectrum, transforms back and stores x < nvx.
+// The unpacking of x0 + i x1 needs Za at -f, which lies on the mirror row: the two
+// rows are done together.  The kernel spectra are real and even, K(-f) = K(f), so
+// only row r's kernel row is read.  Rows past their mirror are skipped.
+template <int NT, int MAXLEN>
+__global__ __launch_bounds__(NT) void k_toep_xfused(StageList F, StageList I, const float2 *__restrict__ za,
+                                                    const float2 *__restrict__ zb, const float2 *__restrict__ K,
+                                                    float2 *__restrict__ wa, float2 *__restrict__ wb, int L1, int L2,
+                                                    int L3, int nvx, FastDiv fL3) {
+    extern __shared__ float2 buf[];
+    const int r = (int)blockIdx.x;
+    const int z = r / L2, y = r - z * L2;
+    const int mr = (z ? L1 - z : 0) * L2 + (y ? L2 - y : 0);
+    if (mr < r) return;
+    const bool self = mr == r;
+    const int nl = self ? 2 : 4;
+    const long long rb = (long long)r * L3, mb = (long long)mr * L3;
+    for (int e = threadIdx.x; e < nl * L3; e += NT) {
+        const int l = (int)fL3.div((unsigned)e), n = e - l * L3;
+        const float2 *src = (l & 1) ? zb : za;
+        buf[e] = n < nvx ? src[(l < 2 ? rb : mb) + n] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    KParams P;
+    P.m = L3;
+    P.L = nl;
+    SubLines g;
+    g.nv = nl;
+    run_stages<NT, MAXLEN>(buf, P, g, F);
+    constexpr int kMaxX = (MAXLEN / 4 + NT - 1) / NT;   // x positions per thread (4 lines of L3 <= MAXLEN / 4)
+    float2 oa[kMaxX], ob[kMaxX], oam[kMaxX], obm[kMaxX];
+#pragma unroll
+    for (int u = 0; u < kMaxX; ++u) {
+        const int x = threadIdx.x + u * NT;
+        if (x < L3) {
+            const int xm = x ? L3 - x : 0;
+            const float2 *kr = K + 3 * (rb + x);
+            float2 X0, X1;
+            if (self) {
+                unpack_pair(buf[x], buf[xm], X0, X1);
+                toep_block(kr[0], kr[1], kr[2], X0, X1, buf[L3 + x], oa[u], ob[u]);
+            } else {
+                unpack_pair(buf[x], buf[2 * L3 + xm], X0, X1);
+                toep_block(kr[0], kr[1], kr[2], X0, X1, buf[L3 + x], oa[u], ob[u]);
+                const float2 *km = K + 3 * (rb + xm);   // K(mr, x) = K(r, -x)
+                unpack_pair(buf[2 * L3 + x], buf[xm], X0, X1);
+                toep_block(km[0], km[1], km[2], X0, X1, buf[3 * L3 + x], oam[u], obm[u]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMaxX; ++u) {
+        const int x = threadIdx.x + u * NT;
+        if (x < L3) {
+            buf[x] = oa[u];
+            buf[L3 + x] = ob[u];
+            if (!self) {
+                buf[2 * L3 + x] = oam[u];
+                buf[3 * L3 + x] = obm[u];
+            }
+        }
+    }
+    __syncthreads();
+    run_stages<NT, MAXLEN>(buf, P, g, I);
+    for (int e = threadIdx.x; e < nl * L3; e += NT) {
+        const int l = (int)fL3.div((unsigned)e), n = e - l * L3;
+        if (n < nvx) ((l & 1) ? wb : wa)[(l < 2 ? rb : mb) + n] = buf[e];
+    }
+}
+
+}  // namespace
+
+void toep_spectral_x(Ctx &cx, const float2 *za, const float2 *zb, const float2 *K, float2 *wa, float2 *wb, int L1,
+                     int L2, int L3, int nvx) {
+    const FftPlan &pf = get_fft_plan(cx, L3, -1), &pi = get_fft_plan(cx, L3, +1);
+    const StageList F = make_stage_list(pf), I = make_stage_list(pi);
+    const long long rows = (long long)L1 * L2;
+    LAM_REQUIRE(rows < (1ll << 31), LAM_ERR_ARG, "Toeplitz grid too large");
+    const size_t lds = sizeof(float2) * 4 * (size_t)L3;
+    TimedScope ts(cx, "toep_xfused");
+    if (4 * L3 <= kLdsComplex) {
+        hipLaunchKernelGGL((k_toep_xfused<kThreads, kLdsComplex>), dim3((unsigned)rows), dim3(kThreads), lds, cx.stream,
+                           F, I, za, zb, K, wa, wb, L1, L2, L3, nvx, FastDiv((unsigned)L3));
+    } else {
+        LAM_REQUIRE(4 * L3 <= kFusedMaxLen, LAM_ERR_ARG, "Toeplitz grid line too long for the fused step");
+        const void *fn = (const void *)&k_toep_xfused<kThreads, kFusedMaxLen>;
+        if (lds > 64 * 1024) LAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_toep_xfused<kThreads, kFusedMaxLen>), dim3((unsigned)rows), dim3(kThreads), lds,
+                           cx.stream, F, I, za, zb, K, wa, wb, L1, L2, L3, nvx, FastDiv((unsigned)L3));
     }
     check_launch();
 }

@@ -1166,97 +1166,6 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
     }
 }
 
-// type 1, step 1, lane per point: each wave of the workgroup owns one (row, tile)
-// chunk and its 2-D box in LDS ([6][ncell]: real and imaginary parts of the 3
-// components, planar), and every lane spreads whole points: W^2 x 6 LDS float
-// atomic adds per point, none of which waits for a result.  Lanes that add at
-// the same instant use the same stamp offset, so two of them meet at one address
-// only when their points share a first cell; lane l therefore takes a contiguous
-// run of the chunk's points (sorted by first cell), ceil(n / 64) of them, so the
-// lanes work on points far apart in that order.  The box then goes to the slab
-// and the halo record exactly as in k_rows_spread.
-constexpr int kRowsSpreadWaves = 4;
-
-template <int W>
-__global__ __launch_bounds__(64 * kRowsSpreadWaves) void k_rows_spread_lp(
-    float2 *__restrict__ slab, float2 *__restrict__ halo, const float2 *__restrict__ c, const Geo *__restrict__ geo,
-    long long P, int R, int C, int nfz, int nfx, float beta, PointFrame fr, RowGeom rg, int j0,
-    const unsigned *__restrict__ ids, const uint4 *__restrict__ chunks, long long nchunks, int paired) {
-    extern __shared__ float lds[];
-    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    const long long ci = (long long)blockIdx.x * kRowsSpreadWaves + wave;
-    const bool live = ci < nchunks;
-    const int ncell = rg.HBz * rg.HBx;
-    const int HB = rg.HBx;
-    float *box = lds + (size_t)wave * 6 * ncell;
-    for (int i = lane; i < 6 * ncell; i += 64) box[i] = 0.f;
-    __syncthreads();
-    const uint4 ch = live ? chunks[ci] : make_uint4(0u, 0u, 0u, 0u);
-    const Box2 bx = box2(ch.x, j0, rg);
-    if (live) {
-        const unsigned n = ch.z, nit = (n + 63u) / 64u;
-        const long long S = (long long)R * C, H = (S + 1) / 2;
-        for (unsigned k = 0; k < nit; ++k) {
-            const unsigned i = (unsigned)lane * nit + k;
-            if (i >= n) continue;
-            const long long gid = ids[ch.y + i];
-            const long long p = gid / H;
-            const long long idx = gid - p * H;
-            const int j = (int)(idx / C), kk = (int)(idx - (long long)j * C);
-            const Geo g = geo[p];
-            float2 vc[3];
-            half_point_values(c, p, idx, S, g, paired, vc);
-            double tx, ty, tz;
-            point_coords(g, fr, j, kk, tx, ty, tz);
-            const int ix = first_cell(tx, nfx, fr.rx, W), iz = first_cell(tz, nfz, fr.rz, W);
-            float wz[W], wx[W];
-            es_weights<W>(tz, iz, beta, wz);
-            es_weights<W>(tx, ix, beta, wx);
-            // the binning placed the point in this tile with the same arithmetic; the
-            // clamp only keeps every LDS access inside the box
-            const int lz = min(max(wrap0(iz, nfz) - bx.oz, 0), rg.Tz - 1);
-            const int lx = min(max(wrap0(ix, nfx) - bx.ox, 0), rg.Tx - 1);
-            float wv[W][6];
-#pragma unroll
-            for (int e = 0; e < W; ++e) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    wv[e][2 * q] = wx[e] * vc[q].x;
-                    wv[e][2 * q + 1] = wx[e] * vc[q].y;
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < W; ++a) {
-                const int rowc = (lz + a) * HB + lx;
-#pragma unroll
-                for (int e = 0; e < W; ++e) {
-#pragma unroll
-                    for (int m = 0; m < 6; ++m) atomicAdd(&box[m * ncell + rowc + e], wz[a] * wv[e][m]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (!live) return;
-    float2 *rs = slab + (long long)bx.r * nfz * nfx * 3;
-    float2 *hrec = halo + ((long long)bx.r * rg.ntiles + bx.tile) * rg.NH * 3;
-    for (int cell = lane; cell < ncell; cell += 64) {
-        const int cz = (int)rg.fbx.div((unsigned)cell), cxx = cell - cz * HB;
-        float2 *dst;
-        if (cz < rg.Tz && cxx < rg.Tx) {
-            dst = rs + ((long long)(bx.oz + cz) * nfx + (bx.ox + cxx)) * 3;
-        } else {
-            int h;
-            if (cz >= rg.Tz && cxx < rg.Tx) h = (cz - rg.Tz) * rg.Tx + cxx;
-            else if (cz < rg.Tz) h = (W - 1) * rg.Tx + cz * (W - 1) + (cxx - rg.Tx);
-            else h = (W - 1) * rg.Tx + rg.Tz * (W - 1) + (cz - rg.Tz) * (W - 1) + (cxx - rg.Tx);
-            dst = hrec + (long long)h * 3;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) dst[q] = make_float2(box[2 * q * ncell + cell], box[(2 * q + 1) * ncell + cell]);
-    }
-}
-
 // ------------------------------------------------------------------ host helpers
 size_t next_smooth_even(size_t n) {
     for (;; ++n) {
@@ -1729,29 +1638,16 @@ void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp
     const PointFrame fr = frame(g, pl);
     const size_t lds = rows_spread_lds(rp.rg, W);
     allow_lds(&k_rows_spread<W>, lds);
-    // LAMINO_ROWS_SPREAD=lanes selects the lane-per-point spreader
-    const char *mode = std::getenv("LAMINO_ROWS_SPREAD");
-    const bool lane_per_point = mode && std::string(mode) == "lanes";
-    const size_t lds_lp = (size_t)kRowsSpreadWaves * 6 * rp.rg.HBz * rp.rg.HBx * sizeof(float);
-    allow_lds(&k_rows_spread_lp<W>, lds_lp);
     const unsigned cgrid = grid_for((size_t)pl.nf[0] * pl.nf[2] * 3, 256, 1u << 20);
     for (int j0 = 0; j0 < rp.Jn; j0 += Jg) {
         const int nrows = std::min(Jg, rp.Jn - j0);
         const long long c0 = rp.row_chunk[j0], c1 = rp.row_chunk[j0 + nrows];
         if (c1 > c0) {
             TimedScope ts(cx, "spread_rows2d");
-            if (lane_per_point) {
-                const long long nwg = (c1 - c0 + kRowsSpreadWaves - 1) / kRowsSpreadWaves;
-                hipLaunchKernelGGL((k_rows_spread_lp<W>), dim3((unsigned)nwg), dim3(64 * kRowsSpreadWaves), lds_lp,
-                                   cx.stream, pl.slab.as<float2>(), pl.halo.as<float2>(), c, g.geo.as<Geo>(), g.P,
-                                   (int)g.R, (int)g.C, pl.nf[0], pl.nf[2], pl.beta, fr, rp.rg, j0,
-                                   rp.ids.as<unsigned>(), rp.chunks.as<uint4>() + c0, c1 - c0, paired ? 1 : 0);
-            } else {
-                hipLaunchKernelGGL((k_rows_spread<W>), dim3((unsigned)(c1 - c0)), dim3(64), lds, cx.stream,
-                                   pl.slab.as<float2>(), pl.halo.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R,
-                                   (int)g.C, pl.nf[0], pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(),
-                                   rp.chunks.as<uint4>() + c0, paired ? 1 : 0);
-            }
+            hipLaunchKernelGGL((k_rows_spread<W>), dim3((unsigned)(c1 - c0)), dim3(64), lds, cx.stream,
+                               pl.slab.as<float2>(), pl.halo.as<float2>(), c, g.geo.as<Geo>(), g.P, (int)g.R,
+                               (int)g.C, pl.nf[0], pl.nf[2], pl.beta, fr, rp.rg, j0, rp.ids.as<unsigned>(),
+                               rp.chunks.as<uint4>() + c0, paired ? 1 : 0);
             check_launch();
         }
         TimedScope ts(cx, "y_expand");

@@ -34,6 +34,12 @@ constexpr long long kNoAllocMaxElems = 1ll << 40;
 // non-finite values -> 0 (include/mask.h:22-32); out[j][k][i] = in[i][j][k]
 void mask_infs_nans(Ctx &cx, long long n, float *a);
 void transpose120_dev(Ctx &cx, const long long d[3], const float *in, float *out);
+// out (pd, zeroed by the caller) = pad3d(transpose(in, {2, 0, 1})) for in [d0][d1][d2]
+void transpose201_pad(Ctx &cx, const long long d[3], const float *in, const long long pd[3], float *out);
+// crop2d SYMMETRIC of P planes R2 x C2 to R x C
+void crop2(Ctx &cx, const float *in, long long P, int R2, int C2, float *out, int R, int C);
+// demagnetising field of a 3-component volume (include/demag.h); work: 3 n complex
+void demag(Ctx &cx, const long long d[3], const float *const m[3], float *const h[3], float2 *work);
 double dot_noalloc(Ctx &cx, long long n, const float *a, const float *b);
 
 // ---- stencils (src/finitediff.cpp:7-71, include/bregman.h:33-36)

@@ -288,6 +288,110 @@ __global__ __launch_bounds__(256) void k_transpose120(const float *__restrict__ 
 }
 }  // namespace
 
+namespace {
+// b[k + o1][i + o2][j + o3] = a[i][j][k]: array::transpose(a, {2, 0, 1})
+// (include/array_ops.h:117-137) placed into a symmetric zero padding (pad3d,
+// include/padding.h:168-204), as src/forward.cpp:123-140 prepares a sample.
+__global__ __launch_bounds__(256) void k_transpose201_pad(const float *__restrict__ a, float *__restrict__ b, int d0,
+                                                          int d1, int d2, int p1, int p2, int o1, int o2, int o3) {
+    const long long total = (long long)d0 * d1 * d2;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(id % d2), j = (int)((id / d2) % d1), i = (int)(id / ((long long)d2 * d1));
+        b[(((long long)(k + o1)) * p1 + (i + o2)) * p2 + (j + o3)] = a[id];
+    }
+}
+
+// out[p][r][c] = in[p][r + o1][c + o2]  (crop2d SYMMETRIC, include/padding.h:134-165)
+__global__ __launch_bounds__(256) void k_crop2(const float *__restrict__ in, float *__restrict__ out, long long P,
+                                               int R, int C, int R2, int C2, int o1, int o2) {
+    const long long total = P * R * C;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(id % C);
+        const long long pr = id / C;
+        const int r = (int)(pr % R);
+        const long long p = pr / R;
+        out[id] = in[(p * R2 + r + o1) * C2 + c + o2];
+    }
+}
+
+// Demagnetising field in Fourier space (include/demag.h:13-79): with k the angular
+// frequencies 2 pi f / n of each axis (f in numpy's fftfreq order),
+// H^(k) = -k (k . m^(k)) / |k|^2, H^(0) = 0.  s holds [n1][n2][n3][3] spectra.
+__global__ __launch_bounds__(256) void k_demag(float2 *__restrict__ s, int n1, int n2, int n3) {
+    const long long total = (long long)n1 * n2 * n3;
+    const double d1 = 2.0 * 3.14159265358979323846 / n1, d2 = 2.0 * 3.14159265358979323846 / n2,
+                 d3 = 2.0 * 3.14159265358979323846 / n3;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(id % n3), b = (int)((id / n3) % n2), a = (int)(id / ((long long)n3 * n2));
+        const float kx = (float)((a < (n1 + 1) / 2 ? a : a - n1) * d1);
+        const float ky = (float)((b < (n2 + 1) / 2 ? b : b - n2) * d2);
+        const float kz = (float)((c < (n3 + 1) / 2 ? c : c - n3) * d3);
+        float2 *v = s + 3 * id;
+        if (id == 0) {
+            v[0] = v[1] = v[2] = make_float2(0.f, 0.f);
+            continue;
+        }
+        const float kinv = 1.0f / (kx * kx + ky * ky + kz * kz);
+        const float2 km = make_float2(kx * v[0].x + ky * v[1].x + kz * v[2].x, kx * v[0].y + ky * v[1].y + kz * v[2].y);
+        v[0] = make_float2(-km.x * kx * kinv, -km.y * kx * kinv);
+        v[1] = make_float2(-km.x * ky * kinv, -km.y * ky * kinv);
+        v[2] = make_float2(-km.x * kz * kinv, -km.y * kz * kinv);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_interleave3(const float *__restrict__ m0, const float *__restrict__ m1,
+                                                     const float *__restrict__ m2, float2 *__restrict__ s, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        s[3 * i] = make_float2(m0[i], 0.f);
+        s[3 * i + 1] = make_float2(m1[i], 0.f);
+        s[3 * i + 2] = make_float2(m2[i], 0.f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_deinterleave3(const float2 *__restrict__ s, float *__restrict__ h0,
+                                                       float *__restrict__ h1, float *__restrict__ h2, long long n,
+                                                       float scale) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        h0[i] = s[3 * i].x * scale;
+        h1[i] = s[3 * i + 1].x * scale;
+        h2[i] = s[3 * i + 2].x * scale;
+    }
+}
+}  // namespace
+
+void transpose201_pad(Ctx &cx, const long long d[3], const float *in, const long long pd[3], float *out) {
+    const long long total = d[0] * d[1] * d[2];
+    LAM_REQUIRE(pd[0] >= d[2] && pd[1] >= d[0] && pd[2] >= d[1], LAM_ERR_ARG, "padding smaller than the volume");
+    hipLaunchKernelGGL(k_transpose201_pad, dim3(grid_for((size_t)total, 256, 1u << 16)), dim3(256), 0, cx.stream, in,
+                       out, (int)d[0], (int)d[1], (int)d[2], (int)pd[1], (int)pd[2], (int)((pd[0] - d[2]) / 2),
+                       (int)((pd[1] - d[0]) / 2), (int)((pd[2] - d[1]) / 2));
+    check_launch();
+}
+
+void crop2(Ctx &cx, const float *in, long long P, int R2, int C2, float *out, int R, int C) {
+    LAM_REQUIRE(R <= R2 && C <= C2, LAM_ERR_ARG, "crop larger than the projections");
+    hipLaunchKernelGGL(k_crop2, dim3(grid_for((size_t)(P * R * C), 256, 1u << 16)), dim3(256), 0, cx.stream, in, out,
+                       P, R, C, R2, C2, (R2 - R) / 2, (C2 - C) / 2);
+    check_launch();
+}
+
+void demag(Ctx &cx, const long long d[3], const float *const m[3], float *const h[3], float2 *work) {
+    const long long n = d[0] * d[1] * d[2];
+    const unsigned g = grid_for((size_t)n, 256, 1u << 16);
+    hipLaunchKernelGGL(k_interleave3, dim3(g), dim3(256), 0, cx.stream, m[0], m[1], m[2], work, n);
+    check_launch();
+    fft3d(cx, work, (int)d[0], (int)d[1], (int)d[2], 3, -1);
+    hipLaunchKernelGGL(k_demag, dim3(g), dim3(256), 0, cx.stream, work, (int)d[0], (int)d[1], (int)d[2]);
+    check_launch();
+    fft3d(cx, work, (int)d[0], (int)d[1], (int)d[2], 3, +1);
+    hipLaunchKernelGGL(k_deinterleave3, dim3(g), dim3(256), 0, cx.stream, work, h[0], h[1], h[2], n,
+                       (float)(1.0 / (double)n));
+    check_launch();
+}
+
 void mask_infs_nans(Ctx &cx, long long n, float *a) {
     hipLaunchKernelGGL(k_mask_nonfinite, dim3(grid_for((size_t)n, 256, 1u << 16)), dim3(256), 0, cx.stream, a, n);
     check_launch();

@@ -21,7 +21,8 @@ struct Toep {
     bool per_grid = false;
     DevBuf khat;                    // float2 [L1][L2][L3][3]: (K00,K01),(K02,K11),(K12,K22) spectra / Ltot
     std::vector<DevBuf> khat_g;     // per-grid spectra (when per_grid)
-    DevBuf za, zb, wa, wb;          // work spectra (complex L-grid each)
+    DevBuf za, zb, wa, wb;          // work spectra (complex L-grid each; wa, wb only without fusion)
+    bool fused = true;              // x transforms + multiply in one pass (toep_spectral_x)
     long long applies = 0;
 };
 

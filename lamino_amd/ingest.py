@@ -300,6 +300,52 @@ def recon(path: str, ctx=None):
     return out, write_outputs(req, out)
 
 
+# ---------------------------------------------------------------- forward simulation
+def forward_simulate(m, angles, gamma: float, out_dims, eps: float = 1e-5, ctx=None) -> np.ndarray:
+    """src/forward.cpp:123-183 on the device: components m (3, d0, d1, d2) as read from
+    TIFF -> transpose {2, 0, 1} -> pad3d by sqrt(2) - 1 -> forward projection on
+    PolarGrid(angles, padded(rows), padded(cols), gamma) -> crop2d to out_dims."""
+    ctx = ctx or default_context()
+    cs = [np.ascontiguousarray(m[i], dtype=np.float32) for i in range(3)]
+    a = np.ascontiguousarray(angles, dtype=np.float64)
+    rows, cols = int(out_dims[0]), int(out_dims[1])
+    out = np.empty((a.size, rows, cols), np.float32)
+    check(ctx.lib.lam_forward_simulate(ctx.h, *[c.ctypes.data_as(ctypes.c_void_p) for c in cs], _dims(cs[0].shape),
+                                       a.ctypes.data_as(_lib.c_dp), a.size, float(gamma), rows, cols, float(eps),
+                                       out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def forward_from_config(path: str, ctx=None):
+    """src/forward.cpp main: [input] data_path + component_1..3 (TIFF), [angles] filename,
+    [orientation] gamma (degrees), [output] basedir / filename and dims = [rows, cols].
+    Writes the projections as a TIFF stack; returns (projections, output path)."""
+    cfg = _toml(path)
+    try:
+        inp, ang, ori, out = cfg["input"], cfg["angles"], cfg["orientation"], cfg["output"]
+        base = inp["data_path"]
+        comps = [inp["component_1"], inp["component_2"], inp["component_3"]]
+        angles_file = ang["filename"]
+        # float gamma * M_PI / 180.0f (src/forward.cpp:44), rounded back to float
+        gamma = float(np.float32(float(np.float32(ori["gamma"])) * math.pi / float(np.float32(180.0))))
+        out_file, out_dir = out["filename"], out["basedir"]
+        dims = out["dims"]
+    except (KeyError, TypeError) as e:
+        raise ConfigError(f"missing key in forward-simulation config: {e}") from e
+    if not (isinstance(dims, list) and len(dims) == 2 and all(isinstance(v, int) for v in dims)):
+        raise ConfigError("Output dimensions must be an array of two integers")
+    os.makedirs(out_dir, exist_ok=True)
+    for c in comps:
+        if not os.path.exists(os.path.join(base, c)):
+            raise ConfigError(f"One or more components do not exist in data path: {base}")
+    angles = read_angles(angles_file)
+    m = [read_tiff(os.path.join(base, c)) for c in comps]
+    proj = forward_simulate(m, angles, gamma, dims, ctx=ctx)
+    dst = os.path.join(out_dir, out_file)
+    write_tiff(dst, proj)
+    return proj, dst
+
+
 def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else argv
     if not argv:
