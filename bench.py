@@ -132,18 +132,28 @@ class Projectors:
         self.m = m
         self.md = self.ctx.to_device(m)
         self.grids = [la.PolarGrid.per_projection(s.alpha, s.gamma, self.det, self.det) for s in work.scans]
+        # the row tiling of the projectors, chosen by timing candidates on the first dataset's
+        # grid (the others share its shapes and take the same tiling)
+        t0 = time.perf_counter()
+        tile, tile_ms = la.autotune_bins(self.grids[0], self.vol)
+        self.tuning = {"row_tiling": [tile[0], tile[2]], "spread_interp_ms": round(tile_ms, 3),
+                       "tuning_s": round(time.perf_counter() - t0, 3)}
         self.projs = [self.ctx.alloc((self.P, self.det, self.det), np.float32) for _ in range(self.G)]
+        self.grid_arr = (ctypes.c_void_p * self.G)(*[g.h for g in self.grids])
+        self.proj_arr = (ctypes.c_void_p * self.G)(*[p.ptr for p in self.projs])
         self.outs = [self.ctx.alloc((3,) + self.vol, np.float32) for _ in range(self.G)]
 
     def step(self):
+        """The forward of every dataset from the volume (one fine-grid transform for all of
+        them, lam_forward_multi), then the adjoint of every dataset's projections."""
         from lamino_amd import _lib
         from lamino_amd.api import _dims
 
         lib, ctx, eps = self.ctx.lib, self.ctx, self.la.api.DEFAULT_EPS
         dims = _dims(self.vol)
+        a, b, c = self.md.comps()
+        _lib.check(lib.lam_forward_multi(ctx.h, self.grid_arr, self.G, dims, eps, a, b, c, self.proj_arr))
         for g in range(self.G):
-            a, b, c = self.md.comps()
-            _lib.check(lib.lam_forward(ctx.h, self.grids[g].h, dims, eps, a, b, c, self.projs[g].ptr))
             x, y, z = self.outs[g].comps()
             _lib.check(lib.lam_adjoint(ctx.h, self.grids[g].h, dims, eps, self.projs[g].ptr, x, y, z))
 
@@ -240,7 +250,7 @@ class Projectors:
                                 "l_grid": [toep_len(n) for n in self.vol]}}
 
     def release(self):
-        self.grids = self.projs = self.outs = self.md = None
+        self.grids = self.projs = self.outs = self.md = self.grid_arr = self.proj_arr = None
 
 
 def roofline(pr, kernels, spread_ms, spread_cnt, workload_key):
@@ -315,6 +325,9 @@ def run_recon(la, synth, work, graph: bool):
                         "rel_diff_vs_plain": _rel(recg, rec)}
         if sg["cg_iters"]:
             out["graph"]["cg_iters_per_s"] = round(sg["cg_iters"] / sg["solve_s"], 2)
+        if work.method == 3:
+            out["graph"]["s_per_iteration"] = round(sg["solve_s"] / max(1, sg["outer_iters"]), 5)
+            out["graph"]["normal_ops"] = sg["normal_ops"]
     return out
 
 
@@ -339,6 +352,13 @@ def run_batched(la, synth, njobs, slots):
     specs = synth.batched_specs(njobs)
     log(f"batched: simulating {len(specs)} jobs")
     jobs = _simulate_jobs(la, synth, specs)
+    # Untimed warm-up: one-iteration copies of the jobs through a graph-form server tune
+    # every job shape's projector tiling (a process-wide table), so that both timed forms
+    # run the same tilings and their results differ only by the setup's summation order.
+    import dataclasses
+    with Server(slots=slots, mode="graph") as srv:
+        for i in [srv.submit(m, ds, rd, dataclasses.replace(prm, max_iters=1)) for m, ds, rd, prm, _ in jobs]:
+            srv.wait(i)
     log("batched: plain form, one job at a time")
     t0 = time.perf_counter()
     plain = [la.api._mbir(m, ds, rd, prm) for m, ds, rd, prm, _ in jobs]
@@ -433,7 +453,9 @@ WORKLOAD_KEY = {1: "plain_cg_89x21x89_det89", 2: "split_bregman_179x43x179_det30
 
 def describe(work, pr):
     return (f"{work.name}; padded volume {list(pr.vol)}, detector {pr.det}x{pr.det}, {pr.G} x {pr.P} projections; "
-            "step = forward + adjoint of all datasets (3 components); point binning built once per grid")
+            "step = forward of all datasets (one shared fine-grid transform) + adjoint of each (3 components); "
+            "row tiling autotuned and point binning "
+            "built once per grid before the timed steps")
 
 
 def main():
@@ -479,8 +501,8 @@ def main():
         cpu = cpu_baseline(pr, args.cpu_projections or {1: 24, 2: 12, 3: 4}[args.config])
         log("cpu:", cpu)
     head = {"workload": describe(work, pr), "padded_volume": list(pr.vol), "detector": pr.det, "datasets": pr.G,
-            "projections": pr.P, "nufft_eps": la.api.DEFAULT_EPS, "point_binning": "cached per grid (built in warmup)",
-            "bin_build_ms": pr.bin_ms}
+            "projections": pr.P, "nufft_eps": la.api.DEFAULT_EPS, "point_binning": "cached per grid (built by the autotuner and in warmup)",
+            "bin_build_ms": pr.bin_ms, "autotune": pr.tuning}
     kernels_ms = {k: round(v["total_ms"] / diag_steps, 4) for k, v in kernels.items()}
     pr.release()
 
@@ -492,7 +514,7 @@ def main():
         p2 = Projectors(la, w)
         ms2, _, _ = p2.timed(max(5, args.steps), args.warmup)
         extra[f"config{ci}"] = {"workload": describe(w, p2), "projections_per_s": round(p2.G * p2.P / (ms2 / 1e3), 2),
-                                "ms_per_step": round(ms2, 4), **p2.cg(args.cg_iters)}
+                                "ms_per_step": round(ms2, 4), "autotune": p2.tuning, **p2.cg(args.cg_iters)}
         log(f"config {ci}:", extra[f"config{ci}"])
         p2.release()
 
@@ -502,7 +524,7 @@ def main():
         for ci in sorted({args.config, *[int(c) for c in args.extra_configs.split(",") if c.strip()]}):
             w = synth.config(ci)
             log(f"reconstruction job of configuration {ci} ({w.name})")
-            recon[f"config{ci}"] = run_recon(la, synth, w, graph=w.method != 3)
+            recon[f"config{ci}"] = run_recon(la, synth, w, graph=True)
             log(recon[f"config{ci}"])
 
     batched = None if args.no_batched else run_batched(la, synth, args.batched_jobs, args.slots)

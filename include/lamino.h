@@ -93,6 +93,16 @@ lam_status lam_adjoint(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], doub
 /* adjoint with accumulation into m (the yT += adjoint(...) loop of src/mbir_*.cpp) */
 lam_status lam_adjoint_accumulate(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps,
                                   const float *proj, float *m0, float *m1, float *m2);
+/* The datasets loop of the MBIR drivers in one call (src/mbir3.cpp:66-83,
+ * src/mbir_sb.cpp:60-78: one forward / adjoint per gamma dataset of one volume):
+ * the de-apodisation and the fine-grid 3-D transform run once for all grids.
+ * lam_forward_multi: proj[k] = forward(m, grids[k]).  lam_adjoint_multi:
+ * m = sum_k adjoint(proj[k], grids[k]) (accumulate: m +=); the grids share the
+ * detector size. */
+lam_status lam_forward_multi(lam_ctx *ctx, lam_grid *const *grids, int ngrids, const int64_t dims[3], double eps,
+                             const float *m0, const float *m1, const float *m2, float *const *proj);
+lam_status lam_adjoint_multi(lam_ctx *ctx, lam_grid *const *grids, int ngrids, const int64_t dims[3], double eps,
+                             const float *const *proj, float *m0, float *m1, float *m2, int accumulate);
 /* sysmat (include/tomocam.h:50-52, src/gradient.cpp:33-89), NUFFT form */
 lam_status lam_sysmat(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *const x[3],
                       float *const y[3]);

@@ -10,5 +10,5 @@ __all__ = [
     "Context", "DeviceArray", "PolarGrid", "beam_dir_vector", "forward", "adjoint", "nufft3d1", "nufft3d2",
     "sysmat", "gradient", "residual", "ToeplitzOperator", "fft2", "RampPreconditioner", "grad_u", "divergence",
     "laplacian", "qggmrf", "dot", "cgsolver", "split_bregman", "nagopt", "lipschitz", "ReconParams", "MBIR1",
-    "MBIR2", "MBIR3", "autotune_bins", "mbir_footprint", "demag",
+    "MBIR2", "MBIR3", "autotune_bins", "mbir_footprint", "demag", "forward_multi", "adjoint_multi",
 ]

@@ -82,6 +82,9 @@ _SIGS = {
     "lam_forward": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, c_fp, c_fp, c_fp, c_fp]),
     "lam_adjoint": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, c_fp, c_fp, c_fp, c_fp]),
     "lam_adjoint_accumulate": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, c_fp, c_fp, c_fp, c_fp]),
+    "lam_forward_multi": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i64p, ctypes.c_double, c_fp, c_fp, c_fp, c_vp]),
+    "lam_adjoint_multi": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_i64p, ctypes.c_double, c_vp, c_fp, c_fp, c_fp,
+                                         ctypes.c_int]),
     "lam_sysmat": (ctypes.c_int, [c_vp, c_vp, c_i64p, ctypes.c_double, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     "lam_toep_create": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.c_int, c_i64p, ctypes.c_int,
                                        ctypes.POINTER(c_vp)]),

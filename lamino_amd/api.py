@@ -268,6 +268,35 @@ def adjoint(proj, pg: PolarGrid, recon_dims, gamma: float | None = None, eps: fl
     return out.download()
 
 
+def _grid_array(grids):
+    arr = (ctypes.c_void_p * len(grids))(*[g.h for g in grids])
+    return arr
+
+
+def forward_multi(magnetization, grids, eps: float = DEFAULT_EPS) -> list:
+    """The datasets loop of src/mbir3.cpp:66-83 for the forward: one projection stack per
+    grid from one volume, with one fine-grid transform for all of them."""
+    ctx = grids[0].ctx
+    m = ctx.to_device(np.asarray(magnetization, dtype=np.float32))
+    outs = [ctx.alloc(g.dims(), np.float32) for g in grids]
+    ptrs = (ctypes.c_void_p * len(grids))(*[o.ptr for o in outs])
+    a, b, c = m.comps()
+    check(ctx.lib.lam_forward_multi(ctx.h, _grid_array(grids), len(grids), _dims(m.shape[1:]), eps, a, b, c, ptrs))
+    return [o.download() for o in outs]
+
+
+def adjoint_multi(projs, grids, recon_dims, eps: float = DEFAULT_EPS) -> np.ndarray:
+    """yT = sum_g adjoint(proj_g, grid_g) (src/mbir3.cpp:66-83) with one fine-grid
+    transform for all datasets."""
+    ctx = grids[0].ctx
+    ps = [ctx.to_device(np.asarray(p, dtype=np.float32)) for p in projs]
+    ptrs = (ctypes.c_void_p * len(ps))(*[p.ptr for p in ps])
+    out = ctx.alloc((3,) + tuple(recon_dims), np.float32)
+    a, b, c = out.comps()
+    check(ctx.lib.lam_adjoint_multi(ctx.h, _grid_array(grids), len(grids), _dims(recon_dims), eps, ptrs, a, b, c, 0))
+    return out.download()
+
+
 def sysmat(x, pg: PolarGrid, gamma: float | None = None, eps: float = DEFAULT_EPS) -> np.ndarray:
     """src/gradient.cpp:33-89 (NUFFT form)."""
     ctx = pg.ctx

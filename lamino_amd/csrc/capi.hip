@@ -200,7 +200,21 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
         }
         lip = L;
         long long calls = 0;
-        nagopt(cx, od, grad, loss, x.v(), pr.max_iters, L, pr.tol, pr.xtol, pr.inner_iters, &outer, &calls);
+        if (mode == MbirMode::Graph || mode == MbirMode::Async) {
+            const long long ops0 = toep->applies;
+            LossTermsFn terms = [&](CVol3 u, double *q, double *sd, double *qpart, double *rpart) {
+                toep_quad_dev(*toep, u, q, qpart);
+                RedTerm t[3] = {{RED_DOT, u.c[0], yT.v().c[0]}, {RED_DOT, u.c[1], yT.v().c[1]},
+                                {RED_DOT, u.c[2], yT.v().c[2]}};
+                reduce_terms_dev(cx, n, t, 3, sd, rpart);
+            };
+            nagopt_graph(cx, od, grad, terms, nds, yTy, x.v(), pr.max_iters, L, pr.tol, pr.xtol, pr.inner_iters,
+                         mode == MbirMode::Graph, &outer, &calls);
+            // the replays do not pass through the host's operator count
+            normal_ops = ops0 + calls;
+        } else {
+            nagopt(cx, od, grad, loss, x.v(), pr.max_iters, L, pr.tol, pr.xtol, pr.inner_iters, &outer, &calls);
+        }
     }
     // crop to the requested dims and download
     const long long m = rd[0] * rd[1] * rd[2];
@@ -505,6 +519,42 @@ lam_status lam_adjoint(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], doub
 lam_status lam_adjoint_accumulate(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *proj,
                                   float *m0, float *m1, float *m2) {
     return adjoint_impl(ctx, grid, dims, eps, proj, m0, m1, m2, true);
+}
+
+static std::vector<Grid *> grid_list(lam_grid *const *grids, int ngrids) {
+    LAM_REQUIRE(grids && ngrids >= 1, LAM_ERR_ARG, "no grids");
+    std::vector<Grid *> gp(ngrids);
+    for (int k = 0; k < ngrids; ++k) {
+        LAM_REQUIRE(grids[k] && grids[k]->impl, LAM_ERR_ARG, "null grid");
+        gp[k] = grids[k]->impl.get();
+    }
+    return gp;
+}
+
+lam_status lam_forward_multi(lam_ctx *ctx, lam_grid *const *grids, int ngrids, const int64_t dims[3], double eps,
+                             const float *m0, const float *m1, const float *m2, float *const *proj) {
+    return guard([&] {
+        need(ctx, "ctx"); need(m0, "m0"); need(m1, "m1"); need(m2, "m2"); need(proj, "proj");
+        const auto gp = grid_list(grids, ngrids);
+        for (int k = 0; k < ngrids; ++k) need(proj[k], "proj[k]");
+        long long d[3];
+        dims_of(dims, d);
+        const float *m[3] = {m0, m1, m2};
+        forward_multi(ctx->impl, gp.data(), ngrids, d, eps, m, proj);
+    });
+}
+
+lam_status lam_adjoint_multi(lam_ctx *ctx, lam_grid *const *grids, int ngrids, const int64_t dims[3], double eps,
+                             const float *const *proj, float *m0, float *m1, float *m2, int accumulate) {
+    return guard([&] {
+        need(ctx, "ctx"); need(m0, "m0"); need(m1, "m1"); need(m2, "m2"); need(proj, "proj");
+        const auto gp = grid_list(grids, ngrids);
+        for (int k = 0; k < ngrids; ++k) need(proj[k], "proj[k]");
+        long long d[3];
+        dims_of(dims, d);
+        float *m[3] = {m0, m1, m2};
+        adjoint_multi(ctx->impl, gp.data(), ngrids, d, eps, proj, m, accumulate != 0);
+    });
 }
 
 lam_status lam_sysmat(lam_ctx *ctx, lam_grid *grid, const int64_t dims[3], double eps, const float *const x[3],

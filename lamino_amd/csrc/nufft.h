@@ -53,6 +53,7 @@ struct RowGeom {
     int NH = 0;                // halo cells per tile: (W-1) Tx + Tz (W-1) + (W-1)^2
     int XB = 1;                // spreader x block (divides Tx)
     FastDiv fbx;               // HBx
+    FastDiv fbx3;              // 3 * HBx: complex values per box row of 3 components
 };
 
 struct RowPlan {
@@ -64,6 +65,8 @@ struct RowPlan {
     DevBuf iy;                         // int [Jn] unwrapped first y cell of each row
     std::vector<int> iy_host;          // the same on the host
     DevBuf count;                      // unsigned [Jn][ntiles] points per (row, tile)
+    DevBuf nz;                         // unsigned [ntiles][nzw]: bit j % 32 of word j / 32 set when (row j, tile) holds points
+    int nzw = 0;
     DevBuf ids;                        // half-point ids sorted by (row, tile, first cell)
     DevBuf chunks;                     // uint4 (row * ntiles + tile, first, count, 0), one per non-empty (row, tile)
     long long nchunks = 0;
@@ -110,6 +113,13 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
              bool accumulate);
 void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const float *const x[3],
                   float *const y[3]);
+// Several grids at once: one fine-grid transform (and de-apodisation) serves them all.
+// forward_multi: proj[k] = forward(m, grid k).  adjoint_multi: m (+)= sum_k adjoint(proj[k],
+// grid k); the grids share the detector size.
+void forward_multi(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], double eps,
+                   const float *const m[3], float *const *proj);
+void adjoint_multi(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], double eps,
+                   const float *const *proj, float *const m[3], bool accumulate);
 // Picks the spreading/interpolation tile shape of this grid and fine grid by
 // timing candidates; the choice is kept in cx.bin_tune.  best_ms: spread+interp time.
 std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms);

@@ -690,6 +690,67 @@ __global__ __launch_bounds__(256) void k_bin_keys_rows(const Geo *__restrict__ g
     }
 }
 
+// The 2-D boxes that reach fine-grid cell (z, x): its own tile's, and for a cell
+// in the first W - 1 rows / columns of its tile the boxes of the tiles above
+// (S), to the left (E) and above-left (C), which extend W - 1 cells past their
+// tile.  A row's slab value at (z, x) is read (interpolation) or written
+// (spreading) only through a box of a tile that holds points of that row.
+struct CellTiles {
+    unsigned t0, tS, tE, tC;
+    int lz, lx;
+    bool south, east;
+};
+
+template <int W>
+__device__ __forceinline__ CellTiles cell_tiles(int z, int x, const RowGeom &rg) {
+    CellTiles c;
+    const int tz = z / rg.Tz, tx = x / rg.Tx;
+    c.lz = z - tz * rg.Tz;
+    c.lx = x - tx * rg.Tx;
+    const int tzu = tz > 0 ? tz - 1 : rg.ntz - 1, txl = tx > 0 ? tx - 1 : rg.ntx - 1;
+    c.t0 = (unsigned)(tz * rg.ntx + tx);
+    c.tS = (unsigned)(tzu * rg.ntx + tx);
+    c.tE = (unsigned)(tz * rg.ntx + txl);
+    c.tC = (unsigned)(tzu * rg.ntx + txl);
+    c.south = c.lz < W - 1;
+    c.east = c.lx < W - 1;
+    return c;
+}
+
+// Which of those boxes hold points in row j, from the plan's bit table ([tile][word],
+// bit j % 32 of word j / 32).  The rows are walked in one direction (dir), so the
+// four words of the next 32-row block are loaded while the current block is used.
+struct NzRow {
+    int blk = -1;
+    unsigned b0 = 0, bS = 0, bE = 0, bC = 0;      // current block
+    unsigned q0 = 0, qS = 0, qE = 0, qC = 0;      // next block (in the walk's direction)
+    bool n0 = false, nS = false, nE = false, nC = false;
+    __device__ __forceinline__ void load(const unsigned *__restrict__ nz, int nzw, const CellTiles &c, int b,
+                                         unsigned &w0, unsigned &wS, unsigned &wE, unsigned &wC) {
+        b = b < 0 ? 0 : (b >= nzw ? nzw - 1 : b);
+        w0 = nz[(long long)c.t0 * nzw + b];
+        wS = c.south ? nz[(long long)c.tS * nzw + b] : 0u;
+        wE = c.east ? nz[(long long)c.tE * nzw + b] : 0u;
+        wC = c.south && c.east ? nz[(long long)c.tC * nzw + b] : 0u;
+    }
+    __device__ __forceinline__ void at(const unsigned *__restrict__ nz, int nzw, const CellTiles &c, int j,
+                                       int dir) {
+        const int b = j >> 5;
+        if (b != blk) {
+            if (blk < 0) load(nz, nzw, c, b, q0, qS, qE, qC);
+            b0 = q0; bS = qS; bE = qE; bC = qC;
+            blk = b;
+            load(nz, nzw, c, b + (dir > 0 ? 1 : -1), q0, qS, qE, qC);
+        }
+        const unsigned sh = (unsigned)(j & 31);
+        n0 = (b0 >> sh) & 1u;
+        nS = (bS >> sh) & 1u;
+        nE = (bE >> sh) & 1u;
+        nC = (bC >> sh) & 1u;
+    }
+    __device__ __forceinline__ bool any() const { return n0 || nS || nE || nC; }
+};
+
 // type 2, step 1: G_r(z, x) = sum_b wy[j][b] F(z, iy[j] + b, x), j = j0 + r.  One thread per
 // (z, x, component) walks the group's rows in the order of non-decreasing iy with
 // a window of W fine-grid values in registers, so each value is loaded once.
@@ -739,8 +800,8 @@ __global__ __launch_bounds__(256) void k_rows_contract(const float2 *__restrict_
 // wrap: the planes need no zeroing and the flushes wait on no load).
 template <int W>
 __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, const float2 *__restrict__ slab,
-                                                     const float2 *__restrict__ halo, const unsigned *__restrict__ count,
-                                                     const float *__restrict__ wy, const int *__restrict__ iy,
+                                                     const float2 *__restrict__ halo, const unsigned *__restrict__ nz,
+                                                     int nzw, const float *__restrict__ wy, const int *__restrict__ iy,
                                                      int nfz, int nfy, int nfx, RowGeom rg, int j0, int nrows,
                                                      int dir, int accumulate) {
     const long long nxc = (long long)nfx * 3, total = (long long)nfz * nxc;
@@ -749,14 +810,11 @@ __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, 
         const int z = (int)(id / nxc);
         const long long xc = id - (long long)z * nxc;
         const int x = (int)(xc / 3), m = (int)(xc - 3ll * x);
-        const int tz = z / rg.Tz, lz = z - tz * rg.Tz, tx = x / rg.Tx, lx = x - tx * rg.Tx;
-        const int tzu = tz > 0 ? tz - 1 : rg.ntz - 1, txl = tx > 0 ? tx - 1 : rg.ntx - 1;
-        const unsigned t0 = (unsigned)(tz * rg.ntx + tx);
-        const bool south = lz < W - 1, east = lx < W - 1;
-        const unsigned tS = (unsigned)(tzu * rg.ntx + tx), tE = (unsigned)(tz * rg.ntx + txl),
-                       tC = (unsigned)(tzu * rg.ntx + txl);
+        const CellTiles ct = cell_tiles<W>(z, x, rg);
+        const int lz = ct.lz, lx = ct.lx;
         const int hS = lz * rg.Tx + lx, hE = (W - 1) * rg.Tx + lz * (W - 1) + lx,
                   hC = (W - 1) * rg.Tx + rg.Tz * (W - 1) + lz * (W - 1) + lx;
+        NzRow nr;
         float2 *col = fine + (long long)z * nfy * nxc + xc;
         int ybase = iy[j0 + (dir > 0 ? 0 : nrows - 1)];
         float2 acc[W];
@@ -774,20 +832,17 @@ __global__ __launch_bounds__(256) void k_rows_expand(float2 *__restrict__ fine, 
                 acc[W - 1] = make_float2(0.f, 0.f);
                 ++ybase;
             }
-            // Every address below lies inside the buffers whether or not its tile holds
-            // points, so all loads are issued at once and the empty tiles' values (never
-            // written) are masked out afterwards instead of branching on the counts.
-            const unsigned *cj = count + (long long)j * rg.ntiles;
+            // only the tiles whose boxes hold points of this row stored values here; the
+            // others' slab and halo cells are not read (a quarter of the (row, tile)
+            // pairs of a polar grid are empty)
+            nr.at(nz, nzw, ct, j, dir);
+            if (!nr.any()) continue;
             const float2 *hr = halo + (long long)r * rg.ntiles * rg.NH * 3;
-            const unsigned n0 = cj[t0], nS = cj[tS], nE = cj[tE], nC = cj[tC];
-            const float2 v0 = slab[((long long)r * nfz + z) * nxc + xc];
-            const float2 vS = hr[((long long)tS * rg.NH + (south ? hS : 0)) * 3 + m];
-            const float2 vE = hr[((long long)tE * rg.NH + (east ? hE : 0)) * 3 + m];
-            const float2 vC = hr[((long long)tC * rg.NH + (south && east ? hC : 0)) * 3 + m];
-            float2 h = n0 ? v0 : make_float2(0.f, 0.f);
-            if (south && nS) h = cadd(h, vS);
-            if (east && nE) h = cadd(h, vE);
-            if (south && east && nC) h = cadd(h, vC);
+            float2 h = make_float2(0.f, 0.f);
+            if (nr.n0) h = slab[((long long)r * nfz + z) * nxc + xc];
+            if (nr.nS) h = cadd(h, hr[((long long)ct.tS * rg.NH + hS) * 3 + m]);
+            if (nr.nE) h = cadd(h, hr[((long long)ct.tE * rg.NH + hE) * 3 + m]);
+            if (nr.nC) h = cadd(h, hr[((long long)ct.tC * rg.NH + hC) * 3 + m]);
             const float *w = wy + (long long)j * W;
 #pragma unroll
             for (int b = 0; b < W; ++b) {
@@ -897,18 +952,23 @@ __global__ __launch_bounds__(kRowsInterpThreads, rows_interp_waves(W)) void k_ro
     // each thread issues the loads of kBoxBatch box values before writing them to
     // LDS (one at a time, the staging waited on each load in turn: 0.43 -> 0.35 ms
     // per launch at the benchmark shape)
+    // Box element i = (cz * HBx + cx) * 3 + q.  A box reaches at most W - 2 cells past the
+    // end of an axis (tile edges divide the axis and are at least W), so one conditional
+    // subtraction wraps it; offsets inside one row's slab fit 32 bits.
     constexpr int kBoxBatch = 8;
-    const int nbox = 3 * bx.ncell;
+    const int nbox = 3 * bx.ncell, rw = 3 * rg.HBx;
     for (int i0 = threadIdx.x; i0 < nbox; i0 += kBoxBatch * kRowsInterpThreads) {
         float2 v[kBoxBatch];
 #pragma unroll
         for (int u = 0; u < kBoxBatch; ++u) {
             const int i = i0 + u * kRowsInterpThreads;
             if (i < nbox) {
-                const int cell = i / 3, q = i - cell * 3;
-                const int cz = (int)rg.fbx.div((unsigned)cell), cxx = cell - cz * rg.HBx;
-                const int gz = wrap_up(bx.oz + cz, nfz), gx = wrap_up(bx.ox + cxx, nfx);
-                v[u] = rs[((long long)gz * nfx + gx) * 3 + q];
+                const int cz = (int)rg.fbx3.div((unsigned)i), e = i - cz * rw;
+                const int cxx = (int)((unsigned)e / 3u), q = e - 3 * cxx;
+                int gz = bx.oz + cz, gx = bx.ox + cxx;
+                gz -= gz >= nfz ? nfz : 0;
+                gx -= gx >= nfx ? nfx : 0;
+                v[u] = rs[(gz * nfx + gx) * 3 + q];
             }
         }
 #pragma unroll
@@ -1101,14 +1161,12 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
             for (int m = 0; m < 3; ++m) *reinterpret_cast<float2 *>(sp + 2 * m) = vc[m];
 #pragma unroll
             for (int a = 0; a < W; ++a) sp[SL::WZ + a] = wz[a];
-            // the x weights shifted by dx into the block's WB columns, zero elsewhere
+            // the x weights shifted by dx into the block's WB columns, zero elsewhere (the
+            // lane's own writes land in order)
 #pragma unroll
-            for (int e = 0; e < SL::WB; ++e) {
-                float v = 0.f;
+            for (int e = 0; e < SL::WB; ++e) sp[SL::WX + e] = 0.f;
 #pragma unroll
-                for (int m = 0; m < W; ++m) v = (e - dx == m) ? wx[m] : v;
-                sp[SL::WX + e] = v;
-            }
+            for (int m = 0; m < W; ++m) sp[SL::WX + dx + m] = wx[m];
         } else if (lane < nb4) {
             // padding records: zero values and weights (LDS may hold any bit pattern)
 #pragma unroll
@@ -1435,22 +1493,22 @@ long long half_rows(const Grid &g) {
 
 // a tile edge that divides n, is a multiple of mult, at least w (the halo reaches one tile
 // only), closest to the target; 0 if there is none
-int row_tile(int n, int w, int mult = 1) {
+int row_tile(int n, int w, int mult, int target) {
     int best = 0;
     for (int d = w; d <= std::min(n, 48); ++d)
-        if (n % d == 0 && d % mult == 0 &&
-            (best == 0 || std::abs(d - kTileRowsTarget) < std::abs(best - kTileRowsTarget)))
+        if (n % d == 0 && d % mult == 0 && (best == 0 || std::abs(d - target) < std::abs(best - target)))
             best = d;
     return best;
 }
 
-RowGeom row_geom(const NufftPlan &pl) {
+// The 2-D tiling of the row path for tile edges near (tz, tx).
+RowGeom row_geom(const NufftPlan &pl, int tz = kTileRowsTarget, int tx = kTileRowsTarget) {
     RowGeom rg;
     const int W = pl.w;
-    rg.Tz = row_tile(pl.nf[0], W);
+    rg.Tz = row_tile(pl.nf[0], W, 1, tz);
     // the spreader's x block must divide Tx: the largest block that has such a tile
     for (rg.XB = row_xblock_max(W); rg.XB >= 1; --rg.XB)
-        if ((rg.Tx = row_tile(pl.nf[2], W, rg.XB)) != 0) break;
+        if ((rg.Tx = row_tile(pl.nf[2], W, rg.XB, tx)) != 0) break;
     if (rg.XB < 1) rg.XB = 1;
     if (!rg.Tz || !rg.Tx) return rg;
     rg.ntz = pl.nf[0] / rg.Tz;
@@ -1460,19 +1518,41 @@ RowGeom row_geom(const NufftPlan &pl) {
     rg.HBx = rg.Tx + W - 1;
     rg.NH = (W - 1) * rg.Tx + rg.Tz * (W - 1) + (W - 1) * (W - 1);
     rg.fbx = FastDiv((unsigned)rg.HBx);
+    rg.fbx3 = FastDiv((unsigned)(3 * rg.HBx));
     return rg;
 }
 
 // The three-component projectors take the row path when every projection
 // shares the tilt (LAMINO_NUFFT_ROWS=0 turns it off), tile edges dividing the
 // fine grid exist and the sort keys fit 32 bits.
+bool rows_fit(const Grid &g, const RowGeom &rg) {
+    return rg.Tz && half_rows(g) * (long long)rg.ntiles * rg.Tz * rg.Tx <= (1ll << 32);
+}
+
 bool use_rows(const Grid &g, const NufftPlan &pl) {
     if (!g.shared_tilt) return false;
     const char *e = std::getenv("LAMINO_NUFFT_ROWS");
     if (e && e[0] == '0') return false;
-    const RowGeom rg = row_geom(pl);
-    if (!rg.Tz) return false;
-    return half_rows(g) * (long long)rg.ntiles * rg.Tz * rg.Tx <= (1ll << 32);
+    return rows_fit(g, row_geom(pl));
+}
+
+// Tile edge targets (z, x) the row-path autotuner times; the first is the untuned default.
+// (Measured, spread + interpolation of one dataset: at (359, 71, 359) / 615 x 615, 360
+// projections, 12 x 15 9.73 ms against 10.11 for the default 16 x 15 and 11.00 for 24 x 20;
+// at (179, 43, 179) / 305, 10 x 15 1.36 ms against 1.41.)
+constexpr int kRowTileCands[][2] = {{16, 15}, {12, 10}, {12, 15}, {10, 15}, {8, 15}, {10, 20},
+                                    {20, 15}, {16, 20}, {20, 20}, {24, 20}};
+
+// The row tiling of this grid's projectors: the tuned one when there is one, else the default.
+std::vector<long long> row_tune_key(const Grid &g, const NufftPlan &pl) { return tune_key(g, pl, 0); }
+
+RowGeom row_geom_for(Ctx &cx, const Grid &g, const NufftPlan &pl) {
+    std::array<int, 3> T;
+    if (lookup_tuned(cx, row_tune_key(g, pl), T)) {
+        const RowGeom rg = row_geom(pl, T[0], T[2]);
+        if (rg.Tz == T[0] && rg.Tx == T[2] && rows_fit(g, rg)) return rg;
+    }
+    return row_geom(pl);
 }
 
 size_t rows_spread_lds(const RowGeom &rg, int w) {
@@ -1483,7 +1563,8 @@ size_t rows_spread_lds(const RowGeom &rg, int w) {
 size_t rows_interp_lds(const RowGeom &rg) { return 3 * (size_t)rg.HBz * rg.HBx * 8; }
 
 RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
-    std::vector<long long> key = {pl.nf[0], pl.nf[1], pl.nf[2], pl.w};
+    const RowGeom rg0 = row_geom_for(cx, g, pl);
+    std::vector<long long> key = {pl.nf[0], pl.nf[1], pl.nf[2], pl.w, rg0.Tz, rg0.Tx};
     auto it = g.rowplans.find(key);
     if (it != g.rowplans.end()) return *it->second;
     TimedScope ts(cx, "bin_points");
@@ -1491,7 +1572,7 @@ RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
     const long long S = g.R * g.C, H = (S + 1) / 2, total = g.P * H;
     LAM_REQUIRE(total < (1ll << 31), LAM_ERR_ARG, "too many points in one polar grid for 32-bit bins");
     auto rp = std::make_unique<RowPlan>();
-    rp->rg = row_geom(pl);
+    rp->rg = rg0;
     const RowGeom &rg = rp->rg;
     LAM_REQUIRE(rg.Tz > 0, LAM_ERR_STATE, "no row tiling for this fine grid");
     rp->Jn = (int)half_rows(g);
@@ -1558,6 +1639,19 @@ RowPlan &get_rowplan(Ctx &cx, Grid &g, NufftPlan &pl) {
     }
     rp->nchunks = (long long)chunks.size();
     rp->row_chunk[rp->Jn] = rp->nchunks;
+    // bit table of the non-empty (row, tile) pairs, tile-major: the y passes test it
+    // per fine-grid cell and row
+    rp->nzw = (rp->Jn + 31) / 32;
+    {
+        std::vector<unsigned> bits((size_t)rg.ntiles * rp->nzw, 0u);
+        for (long long jt = 0; jt < ncount; ++jt)
+            if (hc[jt]) {
+                const long long j = jt / rg.ntiles, t = jt - j * rg.ntiles;
+                bits[(size_t)t * rp->nzw + (size_t)(j >> 5)] |= 1u << (j & 31);
+            }
+        rp->nz.alloc(sizeof(unsigned) * bits.size());
+        upload_now(cx, rp->nz.ptr, bits.data(), sizeof(unsigned) * bits.size());
+    }
     if (!chunks.empty()) {
         rp->chunks.alloc(sizeof(uint4) * chunks.size());
         LAM_HIP(hipMemcpyAsync(rp->chunks.ptr, chunks.data(), sizeof(uint4) * chunks.size(), hipMemcpyHostToDevice,
@@ -1586,14 +1680,19 @@ RowBytes row_path_bytes(const long long dims[3], double eps, long long P, long l
     pl.w = width_for_eps(eps);
     for (int a = 0; a < 3; ++a) pl.nf[a] = (int)nufft_fine_len(dims[a], eps);
     RowBytes b;
-    const RowGeom rg = row_geom(pl);
-    if (!rg.Tz) return b;
+    if (!row_geom(pl).Tz) return b;
     const long long H = (R * C + 1) / 2, Jn = (H - 1) / C + 1, half = P * H;
-    const size_t per_row = row_slab_bytes(pl) + row_halo_bytes(rg);
-    const size_t groups_rows = std::max<size_t>(1, std::min<size_t>((size_t)Jn, slab_budget() / per_row));
-    b.slab = groups_rows * per_row;
-    const size_t cells = (size_t)Jn * rg.ntiles;
-    b.kept = 4 * (size_t)half + 20 * cells + 8 * (size_t)Jn * pl.w;   // ids, counts + chunks, row weights
+    // the largest over the tilings the autotuner may choose
+    for (const auto &t : kRowTileCands) {
+        const RowGeom rg = row_geom(pl, t[0], t[1]);
+        if (!rg.Tz) continue;
+        const size_t per_row = row_slab_bytes(pl) + row_halo_bytes(rg);
+        const size_t groups_rows = std::max<size_t>(1, std::min<size_t>((size_t)Jn, slab_budget() / per_row));
+        b.slab = std::max(b.slab, groups_rows * per_row);
+        const size_t cells = (size_t)Jn * rg.ntiles;
+        // ids, counts + chunks, row weights, the non-empty bit table
+        b.kept = std::max(b.kept, 4 * (size_t)half + 20 * cells + 8 * (size_t)Jn * pl.w + cells / 8 + 4 * rg.ntiles);
+    }
     b.setup = 16 * (size_t)half + ((size_t)16 << 20);                // sort keys / spare ids, sort scratch
     return b;
 }
@@ -1683,7 +1782,7 @@ void spread_rows_w(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, RowPlan &rp
         }
         TimedScope ts(cx, "y_expand");
         hipLaunchKernelGGL((k_rows_expand<W>), dim3(cgrid), dim3(256), 0, cx.stream, pl.fine.as<float2>(),
-                           pl.slab.as<float2>(), pl.halo.as<float2>(), rp.count.as<unsigned>(), rp.wy.as<float>(),
+                           pl.slab.as<float2>(), pl.halo.as<float2>(), rp.nz.as<unsigned>(), rp.nzw, rp.wy.as<float>(),
                            rp.iy.as<int>(), pl.nf[0], pl.nf[1], pl.nf[2], rp.rg, j0, nrows, rp.dir, stores ? 0 : 1);
         check_launch();
     }
@@ -1710,14 +1809,16 @@ void interp_w(Ctx &cx, Grid &g, NufftPlan &pl, float2 *c, const BinParams &bp, S
 }
 
 // paired: c holds detector-pair spectra (det_fft2_real_pairs), three-component form only
-void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, bool paired = false) {
+// add: accumulate into the fine grid's current contents (no zero fill, no stores)
+void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, bool paired = false,
+                   bool add = false) {
     const int nc = half ? 3 : 1;
     if (half && use_rows(g, pl)) {
         RowPlan &rp = get_rowplan(cx, g, pl);
         // One row group whose y window does not wrap writes every plane of the window
         // once, and the transforms that follow read only the window: no zeroing.
-        const bool stores = rows_per_group(rp, pl) >= rp.Jn && rp.ly > 0;
-        if (!stores)
+        const bool stores = !add && rows_per_group(rp, pl) >= rp.Jn && rp.ly > 0;
+        if (!stores && !add)
             LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * 3,
                                    cx.stream));
         TimedScope ts(cx, "spread");
@@ -1726,7 +1827,9 @@ void launch_spread(Ctx &cx, Grid &g, NufftPlan &pl, const float2 *c, bool half, 
     }
     const BinParams bp = bin_params(cx, g, pl, nc);
     SortedPoints &sp = get_bins(cx, g, pl, half, bp);
-    LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc, cx.stream));
+    if (!add)
+        LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * nc,
+                               cx.stream));
     if (sp.nchunks == 0) return;
     TimedScope ts(cx, "spread");
     if (half) {
@@ -2010,6 +2113,72 @@ void adjoint(Ctx &cx, Grid &g, const long long dims[3], double eps, const float 
     crop_deapod(cx, pl, m, nullptr, 3, 1.0f / (float)(g.R * g.C), accumulate);
 }
 
+namespace {
+// The y window shared by several grids' projectors: theirs when all agree, else all planes.
+YWin common_window(Ctx &cx, Grid *const *grids, int ngrids, NufftPlan &pl) {
+    YWin w = y_window(cx, *grids[0], pl);
+    for (int k = 1; k < ngrids; ++k) {
+        const YWin v = y_window(cx, *grids[k], pl);
+        if (v.lo != w.lo || v.len != w.len) return YWin{};
+    }
+    return w;
+}
+}  // namespace
+
+void forward_multi(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], double eps,
+                   const float *const m[3], float *const *proj) {
+    LAM_REQUIRE(ngrids >= 1, LAM_ERR_ARG, "no grids");
+    NufftPlan &pl = get_nufft_plan(cx, dims, eps);
+    ensure_fine(pl, 3);
+    size_t M = 0;
+    for (int k = 0; k < ngrids; ++k) M = std::max(M, (size_t)(grids[k]->P * grids[k]->R * grids[k]->C));
+    cx.work[0].ensure(sizeof(float2) * M);
+    float2 *cbuf = cx.work[0].as<float2>();
+    deapod_pad(cx, pl, m, nullptr, 3);
+    const YWin yw = common_window(cx, grids, ngrids, pl);
+    fine_fft(cx, pl, 3, -1, yw.lo, yw.len);
+    for (int k = 0; k < ngrids; ++k) {
+        Grid &g = *grids[k];
+        launch_interp(cx, g, pl, cbuf, true);
+        const float scale = 1.0f / (float)(g.R * g.C);
+        if (paired_planes(g)) det_ifft2_real_pairs(cx, cbuf, proj[k], (int)g.P, (int)g.R, (int)g.C, +1, scale);
+        else det_fft2(cx, cbuf, false, proj[k], true, (int)g.P, (int)g.R, (int)g.C, +1, scale, cbuf);
+    }
+}
+
+void adjoint_multi(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], double eps,
+                   const float *const *proj, float *const m[3], bool accumulate) {
+    LAM_REQUIRE(ngrids >= 1, LAM_ERR_ARG, "no grids");
+    NufftPlan &pl = get_nufft_plan(cx, dims, eps);
+    ensure_fine(pl, 3);
+    size_t M = 0;
+    for (int k = 0; k < ngrids; ++k) M = std::max(M, (size_t)(grids[k]->P * grids[k]->R * grids[k]->C));
+    cx.work[0].ensure(sizeof(float2) * M);
+    float2 *cbuf = cx.work[0].as<float2>();
+    const YWin yw = common_window(cx, grids, ngrids, pl);
+    // the first grid stores its window (or zero-fills); with different windows the
+    // union is not what the first grid writes, so the whole fine grid starts at zero
+    bool add = false;
+    if (ngrids > 1 && yw.len == 0) {
+        LAM_HIP(hipMemsetAsync(pl.fine.ptr, 0, sizeof(float2) * (size_t)pl.nf[0] * pl.nf[1] * pl.nf[2] * 3,
+                               cx.stream));
+        add = true;
+    }
+    const float scale = 1.0f / (float)(grids[0]->R * grids[0]->C);
+    for (int k = 0; k < ngrids; ++k) {
+        Grid &g = *grids[k];
+        LAM_REQUIRE(g.R * g.C == grids[0]->R * grids[0]->C, LAM_ERR_SHAPE,
+                    "summed adjoints need one detector size (the 1/(R C) scale is shared)");
+        const bool paired = paired_planes(g);
+        if (paired) det_fft2_real_pairs(cx, proj[k], cbuf, (int)g.P, (int)g.R, (int)g.C, -1);
+        else det_fft2(cx, proj[k], true, cbuf, false, (int)g.P, (int)g.R, (int)g.C, -1, 1.0f, cbuf);
+        launch_spread(cx, g, pl, cbuf, true, paired, add);
+        add = true;
+    }
+    fine_fft(cx, pl, 3, +1, yw.lo, yw.len);
+    crop_deapod(cx, pl, m, nullptr, 3, scale, accumulate);
+}
+
 void sysmat_nufft(Ctx &cx, Grid &g, const long long dims[3], double eps, const float *const x[3],
                   float *const y[3]) {
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
@@ -2040,36 +2209,60 @@ bool bins_tuned(Ctx &cx, Grid &g, const long long dims[3], double eps) {
 std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], double eps, double *best_ms) {
     NufftPlan &pl = get_nufft_plan(cx, dims, eps);
     if (use_rows(g, pl)) {
-        // The projectors of this grid take the row path: its tiling follows from the
-        // fine grid, so there is nothing to choose.  Report it as (Tz, W, Tx) with the
-        // time of one spread + interpolation, and mark the grid as tuned.
+        // The projectors of this grid take the row path: time one spread +
+        // interpolation for each distinct 2-D tiling the candidate edges give (each
+        // with its own point bins), keep the fastest's bins and record its tiling
+        // as (Tz, W, Tx) in the row table; the 3-D table gets the default 3-D tile
+        // so that the grid counts as tuned.
         ensure_fine(pl, 3);
         const size_t M = (size_t)(g.P * g.R * g.C);
         cx.work[0].ensure(sizeof(float2) * M);
         float2 *cbuf = cx.work[0].as<float2>();
         LAM_HIP(hipMemsetAsync(cbuf, 0, sizeof(float2) * M, cx.stream));
-        const RowGeom rg = row_geom(pl);
-        std::array<int, 3> T = {rg.Tz, pl.w, rg.Tx};
+        const auto rkey = row_tune_key(g, pl);
         hipEvent_t e0, e1;
         LAM_HIP(hipEventCreate(&e0));
         LAM_HIP(hipEventCreate(&e1));
         const bool timing = cx.timer.on;
         cx.timer.on = false;
-        float t = 0.f;
+        const bool verbose = std::getenv("LAMINO_TUNE_VERBOSE") != nullptr;
+        std::array<int, 3> best = {0, 0, 0};
+        float best_t = 0.f;
+        std::vector<std::pair<int, int>> seen;
+        auto plan_key = [&](int tz, int tx) {
+            return std::vector<long long>{pl.nf[0], pl.nf[1], pl.nf[2], pl.w, tz, tx};
+        };
         try {
-            launch_spread(cx, g, pl, cbuf, true);   // builds the row bins
-            for (int rep = 0; rep < 2; ++rep) {
-                LAM_HIP(hipEventRecord(e0, cx.stream));
-                launch_spread(cx, g, pl, cbuf, true);
-                launch_interp(cx, g, pl, cbuf, true);
-                LAM_HIP(hipEventRecord(e1, cx.stream));
-                LAM_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                LAM_HIP(hipEventElapsedTime(&ms, e0, e1));
-                t = rep == 0 ? ms : std::min(t, ms);
+            for (const auto &cand : kRowTileCands) {
+                const RowGeom rg = row_geom(pl, cand[0], cand[1]);
+                if (!rows_fit(g, rg)) continue;
+                if (std::find(seen.begin(), seen.end(), std::make_pair(rg.Tz, rg.Tx)) != seen.end()) continue;
+                seen.emplace_back(rg.Tz, rg.Tx);
+                cx.bin_tune[rkey] = {rg.Tz, pl.w, rg.Tx};
+                launch_spread(cx, g, pl, cbuf, true);   // builds this tiling's bins
+                float t = 0.f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    LAM_HIP(hipEventRecord(e0, cx.stream));
+                    launch_spread(cx, g, pl, cbuf, true);
+                    launch_interp(cx, g, pl, cbuf, true);
+                    LAM_HIP(hipEventRecord(e1, cx.stream));
+                    LAM_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    LAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    t = rep == 0 ? ms : std::min(t, ms);
+                }
+                if (verbose) std::fprintf(stderr, "row tiling %d x %d: %.3f ms\n", rg.Tz, rg.Tx, t);
+                if (best[0] == 0 || t < best_t) {
+                    if (best[0]) g.rowplans.erase(plan_key(best[0], best[2]));
+                    best = {rg.Tz, pl.w, rg.Tx};
+                    best_t = t;
+                } else {
+                    g.rowplans.erase(plan_key(rg.Tz, rg.Tx));
+                }
             }
         } catch (...) {
             cx.timer.on = timing;
+            cx.bin_tune.erase(rkey);
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             throw;
@@ -2077,15 +2270,18 @@ std::array<int, 3> autotune_bins(Ctx &cx, Grid &g, const long long dims[3], doub
         cx.timer.on = timing;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        LAM_REQUIRE(best[0] != 0, LAM_ERR_STATE, "no row tiling fits this grid");
         const BinParams bp = bin_params(cx, g, pl, 3);
         const std::array<int, 3> T3 = {bp.T[0], bp.T[1], bp.T[2]};
+        cx.bin_tune[rkey] = best;
         cx.bin_tune[tune_key(g, pl, 3)] = T3;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[rkey] = best;
             g_tune[tune_key(g, pl, 3)] = T3;
         }
-        if (best_ms) *best_ms = t;
-        return T;
+        if (best_ms) *best_ms = best_t;
+        return best;
     }
     ensure_fine(pl, 3);
     const size_t M = (size_t)(g.P * g.R * g.C);

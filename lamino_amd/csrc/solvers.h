@@ -104,7 +104,7 @@ long long cg_graph_solve(Ctx &cx, const long long d[3], const OpFn &A, CVol3 b, 
 
 // How an MBIR job drives its solver: Plain is the host-driven loop of the
 // reference (a host read per test), Async keeps the scalars on the device,
-// Graph also replays captured iterations.  MBIR3's line search stays host-driven.
+// Graph also replays captured iterations (MBIR3's line search decided on the device).
 enum class MbirMode { Plain, Async, Graph };
 
 // One reconstruction job end to end on the context's stream (capi.hip).
@@ -120,5 +120,17 @@ size_t mbir_footprint(const lam_dataset *ds, int nds, const long long rd[3], con
 void split_bregman_graph(Ctx &cx, const long long d[3], const OpFn &A, CVol3 yT, CVol3 x0, double lambda, double mu,
                          long long outer_max, long long inner_max, double tol, double xtol, Vol3 x, bool use_graph,
                          CgStats *st, long long *outer_done);
+
+// nagopt in graph form (MBIR3).  The line search's accept/reject and the
+// momentum and step updates run in a one-thread kernel on the device, so one
+// trial (extrapolation, gradient, step, both losses, Armijo test) is enqueued
+// without host reads and captured once; trials after the stop are no-ops on
+// the iterate.  terms(u, q, s, qpart, rpart) enqueues the data-term pieces of
+// loss(u) = sum_g sum_c sqrt(q[3g+c] - 2 s[c] + yTy) into the device arrays
+// q (3 * ngrids) and s (3), with qpart / rpart as its reduction scratch.
+using LossTermsFn = std::function<void(CVol3 u, double *q, double *s, double *qpart, double *rpart)>;
+void nagopt_graph(Ctx &cx, const long long d[3], const OpFn &grad, const LossTermsFn &terms, int ngrids, double yTy,
+                  Vol3 x, long long max_iters, double lipschitz, double tol, double xtol, long long max_inner,
+                  bool use_graph, long long *outer_done, long long *grad_calls);
 
 }  // namespace lam

@@ -286,6 +286,10 @@ void nagopt(Ctx &cx, const long long d[3], const OpFn &grad, const LossFn &loss,
     long long iter = 0, calls = 0;
     for (; iter < max_iters; ++iter) {
         bool converged = false;
+        // loss(x) after the line search is the last trial's ex: x has not changed since
+        // (the same kernels on the same x give the same bits)
+        bool have_ex = false;
+        double ex_last = 0;
         for (long long inner = 0; inner < max_inner; ++inner) {
             const double beta = tnew * (1 / t - 1);
             tnew = 0.5 * (std::sqrt(std::pow(t, 4) + 4 * std::pow(t, 2)) - std::pow(t, 2));
@@ -297,6 +301,8 @@ void nagopt(Ctx &cx, const long long d[3], const OpFn &grad, const LossFn &loss,
             hipLaunchKernelGGL(k_nag_x, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (float)step, y.cv(), g.cv(), x);
             check_launch();
             const double ex = loss(cv(x));
+            have_ex = true;
+            ex_last = ex;
             const double ey = loss(y.cv());
             RedTerm tg[3] = {{RED_SQ, g.v().c[0], nullptr}, {RED_SQ, g.v().c[1], nullptr}, {RED_SQ, g.v().c[2], nullptr}};
             double sg[3];
@@ -322,7 +328,7 @@ void nagopt(Ctx &cx, const long long d[3], const OpFn &grad, const LossFn &loss,
             }
         }
         if (converged) { ++iter; break; }
-        const double e = loss(cv(x));
+        const double e = have_ex ? ex_last : loss(cv(x));
         if (e < tol) { ++iter; break; }
     }
     if (outer_done) *outer_done = iter;
@@ -535,6 +541,169 @@ void split_bregman_graph(Ctx &cx, const long long d[3], const OpFn &A, CVol3 yT,
         st->applies += it * (1 + inner_max);
     }
     if (outer_done) *outer_done = it;
+}
+
+// ------------------------------------------------------------------ NAG, graph form
+namespace {
+
+constexpr int kNagMaxGrids = 8;
+
+// Device scalar block of one graph-form NAG solve.
+struct NagScalars {
+    double t, tnew, step, step0, beta;
+    double qx[3 * kNagMaxGrids], sx[4], qy[3 * kNagMaxGrids], sy[4], sg[4], se[8];
+    double ex, xerr;
+    long long outer, inner, calls;
+    int accepted, done;
+};
+
+// beta and tnew of the next trial (src/nagopt.cpp:151-154)
+__global__ void k_nag_begin(NagScalars *s) {
+    if (s->done) return;
+    const double t = s->t;
+    s->beta = s->tnew * (1 / t - 1);
+    s->tnew = 0.5 * (sqrt(pow(t, 4.0) + 4 * pow(t, 2.0)) - pow(t, 2.0));
+}
+
+__global__ __launch_bounds__(256) void k_nag_y_dev(long long n, const NagScalars *s, CVol3 x, CVol3 xold, Vol3 y) {
+    if (s->done) return;
+    const float beta = (float)s->beta;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y.c[c][i] = x.c[c][i] + (x.c[c][i] - xold.c[c][i]) * beta;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nag_x_dev(long long n, const NagScalars *s, CVol3 y, CVol3 g, Vol3 x) {
+    if (s->done) return;
+    const float step = (float)s->step;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x.c[c][i] = y.c[c][i] - g.c[c][i] * step;
+    }
+}
+
+// The Armijo test and its consequences (src/nagopt.cpp:164-201) from this trial's
+// reductions: the same FP64 expressions, in the same order, as nagopt's host loop.
+__global__ void k_nag_decide(NagScalars *s, int ngrids, double yTy, long long max_iters, long long max_inner,
+                             double tol, double xtol) {
+    if (s->done) return;
+    double ex = 0, ey = 0;
+    for (int g = 0; g < ngrids; ++g)
+        for (int c = 0; c < 3; ++c) ex += sqrt(s->qx[3 * g + c] - 2.0 * s->sx[c] + yTy);
+    for (int g = 0; g < ngrids; ++g)
+        for (int c = 0; c < 3; ++c) ey += sqrt(s->qy[3 * g + c] - 2.0 * s->sy[c] + yTy);
+    const double gy = 0.5 * s->step * (sqrt(s->sg[0]) + sqrt(s->sg[1]) + sqrt(s->sg[2]));
+    s->calls += 1;
+    s->accepted = 0;
+    bool end = false;
+    if (ex > ey + gy) {
+        s->step *= 0.9;
+        end = s->inner + 1 >= max_inner;
+    } else {
+        s->step = s->step0;
+        s->t = s->tnew;
+        const double xerr = (sqrt(s->se[0]) + sqrt(s->se[1]) + sqrt(s->se[2])) /
+                            (sqrt(s->se[3]) + sqrt(s->se[4]) + sqrt(s->se[5]));
+        s->xerr = xerr;
+        if (xerr < xtol) {
+            s->done = 1;
+            s->outer += 1;
+            return;
+        }
+        s->accepted = 1;
+        end = true;
+    }
+    if (end) {
+        // loss(x) after the line search is this trial's ex (x is unchanged since)
+        s->ex = ex;
+        s->inner = 0;
+        s->outer += 1;
+        if (ex < tol || s->outer >= max_iters) s->done = 1;
+    } else {
+        s->inner += 1;
+    }
+}
+
+// xold = x after an accepted trial
+__global__ __launch_bounds__(256) void k_nag_keep(long long n, const NagScalars *s, CVol3 x, Vol3 xold) {
+    if (s->done || !s->accepted) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xold.c[c][i] = x.c[c][i];
+    }
+}
+
+}  // namespace
+
+void nagopt_graph(Ctx &cx, const long long d[3], const OpFn &grad, const LossTermsFn &terms, int ngrids, double yTy,
+                  Vol3 x, long long max_iters, double lipschitz, double tol, double xtol, long long max_inner,
+                  bool use_graph, long long *outer_done, long long *grad_calls) {
+    LAM_REQUIRE(ngrids >= 1 && ngrids <= kNagMaxGrids, LAM_ERR_ARG, "graph-form NAG takes 1 to 8 datasets");
+    const long long n = d[0] * d[1] * d[2];
+    Vol3Buf xold, y, g;
+    xold.alloc(n); y.alloc(n); g.alloc(n);
+    DevBuf scal(sizeof(NagScalars)), red(sizeof(double) * kRedPartials), qpart(sizeof(double) * toep_quad_partials());
+    NagScalars h{};
+    h.t = h.tnew = 1;
+    h.step = h.step0 = 0.1 / lipschitz;
+    h.done = max_iters <= 0 || max_inner <= 0;
+    upload_now(cx, scal.ptr, &h, sizeof(h));
+    NagScalars *s = scal.as<NagScalars>();
+    copy3(cx, n, cv(x), xold.v());
+    RedTerm tg[3], te[6];
+    for (int c = 0; c < 3; ++c) {
+        tg[c] = RedTerm{RED_SQ, g.v().c[c], nullptr};
+        te[c] = RedTerm{RED_DIFF2, x.c[c], xold.v().c[c]};
+        te[3 + c] = RedTerm{RED_SQ, x.c[c], nullptr};
+    }
+    // one trial of the line search (src/nagopt.cpp:148-201)
+    auto trial = [&] {
+        hipLaunchKernelGGL(k_nag_begin, dim3(1), dim3(1), 0, cx.stream, s);
+        check_launch();
+        hipLaunchKernelGGL(k_nag_y_dev, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (const NagScalars *)s, cv(x),
+                           xold.cv(), y.v());
+        check_launch();
+        grad(y.cv(), g.v());
+        hipLaunchKernelGGL(k_nag_x_dev, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (const NagScalars *)s, y.cv(),
+                           g.cv(), x);
+        check_launch();
+        terms(cv(x), s->qx, s->sx, qpart.as<double>(), red.as<double>());
+        terms(y.cv(), s->qy, s->sy, qpart.as<double>(), red.as<double>());
+        reduce_terms_dev(cx, n, tg, 3, s->sg, red.as<double>());
+        reduce_terms_dev(cx, n, te, 6, s->se, red.as<double>());
+        hipLaunchKernelGGL(k_nag_decide, dim3(1), dim3(1), 0, cx.stream, s, ngrids, yTy, max_iters, max_inner, tol,
+                           xtol);
+        check_launch();
+        hipLaunchKernelGGL(k_nag_keep, dim3(vgrid(n)), dim3(256), 0, cx.stream, n, (const NagScalars *)s, cv(x),
+                           xold.v());
+        check_launch();
+    };
+    auto read = [&] {
+        NagScalars r;
+        LAM_HIP(hipMemcpyAsync(&r, scal.ptr, sizeof(r), hipMemcpyDeviceToHost, cx.stream));
+        LAM_HIP(hipStreamSynchronize(cx.stream));
+        return r;
+    };
+    GraphStep step;
+    // Every trial advances the outer count by at most one, so max_iters - outer trials
+    // never run past the iteration limit; a tolerance can stop earlier, so with one the
+    // host reads the stop flag every kCheck trials.
+    constexpr long long kCheck = 4;
+    const long long max_trials = h.done ? 0 : max_iters * max_inner;
+    long long trials = 0, outer_read = 0;
+    while (trials < max_trials) {
+        long long batch = max_iters - outer_read;
+        if (tol > 0 || xtol > 0) batch = std::min(batch, kCheck);
+        batch = std::min(std::max(batch, 1ll), max_trials - trials);
+        for (long long b = 0; b < batch; ++b, ++trials) step.run(cx, use_graph, trial);
+        const NagScalars r = read();
+        if (r.done) break;
+        outer_read = r.outer;
+    }
+    const NagScalars r = read();
+    if (outer_done) *outer_done = r.outer;
+    if (grad_calls) *grad_calls = r.calls;
 }
 
 }  // namespace lam

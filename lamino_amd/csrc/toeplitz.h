@@ -32,8 +32,11 @@ int toep_len(long long n);
 std::unique_ptr<Toep> toep_create(Ctx &cx, Grid *const *grids, int ngrids, const long long dims[3], bool per_grid);
 // y = sum_g sysmat_g(x) + mu * laplacian(x)
 void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu);
-// q[g*3+i] = x_i . sysmat_g(x)_i  (device FP64 array of 3*ngrids; no sync)
-void toep_quad_dev(Toep &t, CVol3 x, double *q_dev);
+// q[g*3+i] = x_i . sysmat_g(x)_i  (device FP64 array of 3*ngrids; no sync).  partials:
+// device doubles for the block sums (toep_quad_partials() of them), or null for the
+// context's shared reduction scratch (whose address may change; graphs pass their own)
+void toep_quad_dev(Toep &t, CVol3 x, double *q_dev, double *partials = nullptr);
+size_t toep_quad_partials();
 void toep_quad(Toep &t, CVol3 x, double *q_host);
 
 }  // namespace lam

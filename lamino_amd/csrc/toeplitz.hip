@@ -449,20 +449,25 @@ void toep_apply(Toep &t, CVol3 x, Vol3 y, float mu) {
     ++t.applies;
 }
 
-void toep_quad_dev(Toep &t, CVol3 x, double *q_dev) {
+constexpr int kQuadBlocks = 1024;
+size_t toep_quad_partials() { return (size_t)kQuadBlocks * kMaxQuadGrids * 3; }
+
+void toep_quad_dev(Toep &t, CVol3 x, double *q_dev, double *partials) {
     Ctx &cx = *t.cx;
     LAM_REQUIRE(t.per_grid, LAM_ERR_STATE, "Toeplitz operator was built without per-grid kernels");
     forward_spectra(t, x);
     QuadSpec s;
     s.G = t.ngrids;
     for (int g = 0; g < kMaxQuadGrids; ++g) s.K[g] = g < t.ngrids ? t.khat_g[g].as<float2>() : nullptr;
-    const int nb = 1024;
-    cx.red.ensure(sizeof(double) * nb * kMaxQuadGrids * 3);
+    const int nb = kQuadBlocks;
+    if (!partials) {
+        cx.red.ensure(sizeof(double) * toep_quad_partials());
+        partials = cx.red.as<double>();
+    }
     hipLaunchKernelGGL(k_toep_quad, dim3(nb), dim3(256), 0, cx.stream, t.za.as<float2>(), t.zb.as<float2>(), s,
-                       t.L[0], t.L[1], t.L[2], cx.red.as<double>());
+                       t.L[0], t.L[1], t.L[2], partials);
     check_launch();
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, cx.stream, cx.red.as<double>(), nb, 3 * t.ngrids,
-                       q_dev);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, cx.stream, partials, nb, 3 * t.ngrids, q_dev);
     check_launch();
 }
 

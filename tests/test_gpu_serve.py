@@ -62,6 +62,43 @@ def test_server_modes_match_plain_and_oracle(ctx, orc, mode, method):
     assert rel(x, xo) < REC_TOL
 
 
+@pytest.mark.parametrize("mode", ["graph", "async"])
+@pytest.mark.parametrize("case", ["rejections", "xtol"])
+def test_mbir3_device_line_search_follows_plain(ctx, orc, mode, case):
+    """MBIR3's graph form decides the Armijo test on the device.  The reference's test
+    (un-squared norms, src/nagopt.cpp:164-171) rejects trials once the step is about
+    2.4/L (mu = L / 24 as the Lipschitz constant; the iterate then grows, so 4
+    iterations): the step shrinks and the momentum waits.  With xtol = 0.1 and a step
+    of 0.4/L the solve stops at its x-change test before max_iters.  The trials, outer
+    iterations and iterate follow the plain (host-driven) form's and the oracle's."""
+    import lamino_amd as la
+    from lamino_amd.serve import Server
+
+    ds, rd = _job(31, nds=2)
+    _, s0 = la.api._mbir(3, ds, rd, la.ReconParams(max_iters=1, mu=0.0, sigma=1000.0, p=1.2), stats=True)
+    L = s0["lipschitz"]
+    if case == "rejections":
+        prm = la.ReconParams(max_iters=4, inner_iters=5, sigma=1000.0, p=1.2, mu=L / 24, tol=0.0, xtol=0.0)
+    else:
+        prm = la.ReconParams(max_iters=10, inner_iters=3, sigma=1000.0, p=1.2, mu=L / 4, tol=0.0, xtol=0.1)
+    plain, pst = la.api._mbir(3, ds, rd, prm, stats=True)
+    with Server(slots=1, mode=mode) as s:
+        x, info = s.wait(s.submit(3, ds, rd, prm))
+    st = info["stats"]
+    say(f"MBIR3 {mode} {case} L {L:.4g}: vs plain {rel(x, plain):.2e}; plain {pst['normal_ops']} ops "
+        f"{pst['outer_iters']} outer, {mode} {st['normal_ops']} ops {st['outer_iters']} outer")
+    assert info["status"] == 0 and np.isfinite(x).all() and np.isfinite(plain).all()
+    if case == "rejections":
+        assert pst["normal_ops"] > pst["outer_iters"]          # some trials were rejected
+    else:
+        assert pst["outer_iters"] < prm.max_iters              # stopped by the x change
+    assert st["normal_ops"] == pst["normal_ops"] and st["outer_iters"] == pst["outer_iters"]
+    assert rel(x, plain) < MIX_TOL
+    kw = dict(max_iters=prm.max_iters, inner_iters=prm.inner_iters, lam=prm.lam, mu=prm.mu, sigma=prm.sigma,
+              p=prm.p, tol=prm.tol, xtol=prm.xtol)
+    assert rel(x, orc.mbir(3, ds, rd, **kw)) < REC_TOL
+
+
 def test_result_same_alone_and_in_mix(ctx):
     from lamino_amd.serve import Server
 
@@ -154,6 +191,33 @@ def test_autotuned_tiles_keep_projector_parity(ctx, orc):
     y1 = la.forward(m, pg)
     assert rel(a1, a0) < 1e-5 and rel(y1, y0) < 1e-5
     assert rel(a1, orc.adjoint(proj, dims, al, ga)) < 1e-4
+
+
+def test_row_path_autotuner_chooses_a_tiling_and_keeps_parity(ctx, orc):
+    """On the row path (projections sharing the tilt) the autotuner times several 2-D
+    tilings of the (z, x) fine plane, each with its own point bins, and keeps the
+    fastest.  Any tiling changes only FP32 summation order."""
+    import lamino_amd as la
+
+    dims, R, C, P = (89, 21, 89), 89, 89, 24
+    rng = np.random.default_rng(67)
+    al = np.full(P, np.radians(30.0))
+    ga = 2 * np.pi * np.arange(P) / P
+    pg = la.PolarGrid.per_projection(al, ga, R, C)
+    proj = rng.standard_normal((P, R, C)).astype(np.float32)
+    m = rng.standard_normal((3,) + dims).astype(np.float32)
+    a0 = la.adjoint(proj, pg, dims)
+    y0 = la.forward(m, pg)
+    T, ms = la.autotune_bins(pg, dims)
+    say("row tiling (Tz, W, Tx)", T, f"{ms:.3f} ms")
+    nf = 180   # fine length of 89 voxels (2 * 89 -> the next even 2,3,5-smooth length)
+    assert T[1] == 6 and ms > 0
+    assert nf % T[0] == 0 and nf % T[2] == 0 and T[2] % 5 == 0 and min(T[0], T[2]) >= 6
+    a1 = la.adjoint(proj, pg, dims)
+    y1 = la.forward(m, pg)
+    assert rel(a1, a0) < 1e-5 and rel(y1, y0) < 1e-5
+    assert rel(a1, orc.adjoint(proj, dims, al, ga)) < 1e-4
+    assert rel(y1, orc.forward(m, al, ga, R, C)) < 1e-4
 
 
 @pytest.mark.parametrize("cls", ["[16,64,64] MBIR1 1x90", "[32,128,128] MBIR2 2x180",
