@@ -293,12 +293,10 @@ __device__ __forceinline__ long long box_to_fine(const TileBox &b, const BinPara
 // Z = P2m + i P2m+1 in plane 2m: the even plane's value is Z(q) + conj Z(-q),
 // the odd one's -i (Z(q) - conj Z(-q)); the centre point (its own mirror)
 // takes Re Z or Im Z.
-__device__ __forceinline__ void half_point_values(const float2 *__restrict__ c, long long p, long long idx,
-                                                  long long S, const Geo &g, int paired, float2 vc[3]) {
+// a = c(q) and b = c(-q) of the point's (paired) plane, as loaded by half_point_values
+__device__ __forceinline__ void half_point_combine(float2 a, float2 b, long long p, long long idx, long long S,
+                                                   const Geo &g, int paired, float2 vc[3]) {
     const long long mir = S - 1 - idx;
-    const long long pl = paired ? (p & ~1ll) : p;
-    const float2 a = c[pl * S + idx];
-    const float2 b = mir != idx ? c[pl * S + mir] : make_float2(0.f, 0.f);
     float2 v;
     if (!paired) v = mir != idx ? cadd(a, cconj(b)) : a;
     else if (!(p & 1)) v = mir != idx ? make_float2(a.x + b.x, a.y - b.y) : make_float2(a.x, 0.f);
@@ -306,6 +304,15 @@ __device__ __forceinline__ void half_point_values(const float2 *__restrict__ c, 
     vc[0] = cscale(v, (float)g.d0);
     vc[1] = cscale(v, (float)g.d1);
     vc[2] = cscale(v, (float)g.d2);
+}
+
+__device__ __forceinline__ void half_point_values(const float2 *__restrict__ c, long long p, long long idx,
+                                                  long long S, const Geo &g, int paired, float2 vc[3]) {
+    const long long mir = S - 1 - idx;
+    const long long pl = paired ? (p & ~1ll) : p;
+    const float2 a = c[pl * S + idx];
+    const float2 b = mir != idx ? c[pl * S + mir] : make_float2(0.f, 0.f);
+    half_point_combine(a, b, p, idx, S, g, paired, vc);
 }
 
 // The interpolated value of a half-set point and its mirror: s = sum_i d_i c_i
