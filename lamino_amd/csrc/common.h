@@ -219,6 +219,18 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// The hardware deals workgroups to the 8 XCDs (each with its own L2) in turn by
+// linear id, so neighbouring work items, which often share cache lines (the
+// column tiles of an inner-index FFT pass, the overlapping boxes of adjacent 2-D
+// tiles), would each land on a different XCD and each fetch the shared lines from
+// HBM.  xcd_order(lin, total) renumbers so that each XCD takes a contiguous run of
+// work items: a bijection on [0, total) (the last total % 8 keep their numbers).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_order(unsigned lin, unsigned total) {
+    const unsigned full = total / kXcds * kXcds;
+    return lin < full ? (lin % kXcds) * (full / kXcds) + lin / kXcds : lin;
+}
+
 // FP64 block reduction of one value per thread (blockDim.x == 256); result in thread 0.
 __device__ __forceinline__ double block_sum_256(double v) {
     __shared__ double sh[256];

@@ -310,14 +310,27 @@ __device__ __forceinline__ bool in_window(int m, int lo, int len, int n) {
     return d < len;
 }
 
+// The tile of this workgroup, numbered per XCD (xcd_order): consecutive tiles along
+// the contiguous index share cache lines in the inner-index passes (four 615-point
+// columns are 32 bytes of each 128-byte row segment).  Measured at (359, 71, 359) /
+// 615: FFT 12.23 -> 10.60 ms per projector step, CG 3.66 -> 3.38 ms per iteration;
+// results unchanged (the tiles are only renumbered).
+__device__ __forceinline__ uint3 tile_block() {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned total = gx * gy * gridDim.z;
+    const unsigned t = xcd_order(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), total);
+    const unsigned q = t / gx;
+    return make_uint3(t - q * gx, q % gy, q / gy);
+}
+
 // Global coordinates of tile line l; false when the line lies past the set.
-__device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, int ibase, long long &o1, int &o2,
-                                          int &i) {
+__device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, int ibase, int bz, long long &o1,
+                                          int &o2, int &i) {
     const int lo = (int)P.fLi.div((unsigned)l), li = l - lo * P.Li;
     o2 = o2base + lo;
     i = ibase + li;
     if (o2 >= P.a.c2 || i >= P.a.ci) return false;
-    o1 = remap((int)blockIdx.z, P.a.o1_split, P.a.o1_gap);
+    o1 = remap(bz, P.a.o1_split, P.a.o1_gap);
     o2 = remap(o2, P.a.o2_split, P.a.o2_gap);
     return true;
 }
@@ -327,12 +340,13 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
     extern __shared__ float2 buf[];
     const int L = P.L, M = P.m, N = P.n;
     const AxisPass &a = P.a;
-    const int o2base = INNER ? (int)blockIdx.y * P.Lo : (int)blockIdx.x * P.Lo;
-    const int ibase = INNER ? (int)blockIdx.x * P.Li : (int)blockIdx.y;
+    const uint3 bk = tile_block();
+    const int o2base = INNER ? (int)bk.y * P.Lo : (int)bk.x * P.Lo;
+    const int ibase = INNER ? (int)bk.x * P.Li : (int)bk.y;
     const int nv = a.nv < 0 ? N : a.nv;
     const int ns = a.ns < 0 ? N : a.ns;
     // the pair modes' second plane exists for this workgroup's o1 (uniform over the tile)
-    const int o1w = remap((int)blockIdx.z, a.o1_split, a.o1_gap);
+    const int o1w = remap((int)bk.z, a.o1_split, a.o1_gap);
     const bool has2 = o1w < a.c1_in2, put2 = o1w < a.c1_out2;
 
     // 64-bit offsets of the tile's lines, once per workgroup (-1: line past the set),
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
         long long o1;
         int o2, i;
-        const bool ok = tile_line(P, l, o2base, ibase, o1, o2, i);
+        const bool ok = tile_line(P, l, o2base, ibase, (int)bk.z, o1, o2, i);
         base_in[l] = ok ? o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi : -1;
         base_out[l] = ok ? o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi : -1;
     }
