@@ -1097,8 +1097,8 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
     extern __shared__ float lds[];
     const uint4 ch = chunks[blockIdx.x];
     const Box2 bx = box2(ch.x, j0, rg);
-    float *stg = lds;                                                        // [kStageRows][SW]
-    float2 *acc = reinterpret_cast<float2 *>(lds + (size_t)kStageRows * SW);  // [3][ncell]
+    float *stg = lds;                                                        // [kStageRows + 2][SW]
+    float2 *acc = reinterpret_cast<float2 *>(lds + (size_t)(kStageRows + 2) * SW);  // [3][ncell]
     for (int i = threadIdx.x; i < 3 * bx.ncell; i += blockDim.x) acc[i] = make_float2(0.f, 0.f);
     const int lane = threadIdx.x;
     const long long S = (long long)R * C;
@@ -1122,7 +1122,9 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
     for (int q = 0; q < NCH; ++q)
 #pragma unroll
         for (int m = 0; m < 3; ++m) run[q][m] = make_float2(0.f, 0.f);
-    int run_cell = -1;   // wave-uniform
+    // wave-uniform; the first run start flushes the zero run registers into cell 0,
+    // which adds exact zeros, so no test of a "no run yet" state is needed
+    int run_cell = 0;
     auto flush = [&]() {
         float2 t[NCH][3];
 #pragma unroll
@@ -1184,18 +1186,19 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
             for (int i = 0; i < SL::N; ++i) sp[i] = 0.f;
         }
         const int up = __shfl_up(cell, 1);
-        const int prev = lane == 0 ? run_cell : up;
+        const int prev = lane == 0 ? (base == 0 ? -1 : run_cell) : up;
         const unsigned long long starts = __ballot(lane < nb && cell != prev);
         __syncthreads();
 
-        // records are fetched two ahead of the one being added (indices past the batch
-        // wrap into it: those fetches are never used)
+        // records are fetched two ahead of the one being added (the two rows past the
+        // staging area's last are fetched, never written and never used, so that the
+        // fetch addresses are immediate offsets from one base per group of four)
         struct Rec {
             float2 v[3];
             float wz[NCH], wx[NCH];
         };
         auto fetch = [&](int kk, Rec &r) {
-            const float *rp = stg + (kk & (kStageRows - 1)) * SW;
+            const float *rp = stg + kk * SW;
 #pragma unroll
             for (int m = 0; m < 3; ++m) r.v[m] = *reinterpret_cast<const float2 *>(rp + 2 * m);
 #pragma unroll
@@ -1204,9 +1207,9 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
                 r.wx[q] = rp[wxo[q]];
             }
         };
-        auto add = [&](int kk, const Rec &r) {
-            if (starts & (1ull << kk)) {
-                if (run_cell >= 0) flush();
+        auto add = [&](bool start, int kk, const Rec &r) {
+            if (start) {
+                flush();
                 run_cell = __builtin_amdgcn_readlane(cell, kk);
 #pragma unroll
                 for (int q = 0; q < NCH; ++q)
@@ -1227,17 +1230,19 @@ __global__ __launch_bounds__(64) void k_rows_spread(float2 *__restrict__ slab, f
         fetch(0, r0);
         fetch(1, r1);
         for (int kk = 0; kk < nb4; kk += 4) {
+            // the group's four run-start bits, tested against constants
+            const unsigned st = (unsigned)(starts >> kk);
             fetch(kk + 2, r2);
-            add(kk, r0);
+            add(st & 1u, kk, r0);
             fetch(kk + 3, r3);
-            add(kk + 1, r1);
+            add(st & 2u, kk + 1, r1);
             fetch(kk + 4, r0);
-            add(kk + 2, r2);
+            add(st & 4u, kk + 2, r2);
             fetch(kk + 5, r1);
-            add(kk + 3, r3);
+            add(st & 8u, kk + 3, r3);
         }
     }
-    if (run_cell >= 0) flush();
+    flush();
     __syncthreads();
     float2 *rs = slab + (long long)bx.r * nfz * nfx * 3;
     float2 *hrec = halo + ((long long)bx.r * rg.ntiles + bx.tile) * rg.NH * 3;
@@ -1569,7 +1574,7 @@ RowGeom row_geom_for(Ctx &cx, const Grid &g, const NufftPlan &pl) {
 size_t rows_spread_lds(const RowGeom &rg, int w) {
     const int n = 6 + w + (w + row_xblock_max(w) - 1);
     const size_t sw = (size_t)(n + (6 - n % 4) % 4);   // RowStage<w>::SW
-    return (size_t)kStageRows * sw * 4 + 3 * (size_t)rg.HBz * rg.HBx * 8;
+    return (size_t)(kStageRows + 2) * sw * 4 + 3 * (size_t)rg.HBz * rg.HBx * 8;
 }
 size_t rows_interp_lds(const RowGeom &rg) { return 3 * (size_t)rg.HBz * rg.HBx * 8; }
 
