@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--batched-jobs", type=int, default=16)
     ap.add_argument("--served-jobs", type=int, default=12)
     ap.add_argument("--slots", type=int, default=4, help="concurrent jobs of the batched/served server")
+    ap.add_argument("--batched-plain-only", action="store_true",
+                    help="run only the batched mix's jobs one at a time (the plain form), for a kernel trace "
+                         "of the GPU's busy time in that form")
     ap.add_argument("--profile", action="store_true",
                     help="only the headline's timed fwd+adj steps, a few CG iterations and a 1 GiB "
                          "device-to-device copy (PMC calibration), for rocprofv3 runs")
@@ -382,6 +385,20 @@ def run_batched(la, synth, njobs, slots):
     return out
 
 
+def run_batched_plain(la, synth, njobs):
+    """The batched mix's jobs one at a time (the plain form) and nothing else.  A 3x5x7
+    fft2 just before the pass marks its start in a kernel trace; the GPU's busy time is the
+    union of the kernel intervals after it."""
+    specs = synth.batched_specs(njobs)
+    jobs = _simulate_jobs(la, synth, specs)
+    la.default_context().sync()
+    la.fft2(np.ones((3, 5, 7), np.complex64), -1)
+    t0 = time.perf_counter()
+    for m, ds, rd, prm, _ in jobs:
+        la.api._mbir(m, ds, rd, prm)
+    log(f"batched mix, plain form: {time.perf_counter() - t0:.3f} s")
+
+
 def run_served(la, synth, njobs, slots):
     """Continuous batching under Poisson arrivals: jobs are submitted at their arrival
     times while others run; latency = arrival -> result on host."""
@@ -467,6 +484,10 @@ def main():
 
     import lamino_amd as la
     from lamino_amd import _lib, synth
+
+    if args.batched_plain_only:
+        run_batched_plain(la, synth, args.batched_jobs)
+        return
 
     work = synth.config(args.config)
     pr = Projectors(la, work)
