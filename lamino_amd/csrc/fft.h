@@ -34,15 +34,17 @@ enum { ST_COMPLEX = 0, ST_REAL = 1, ST_REAL_PAIR = 2, ST_IMAG = 3 };
 struct FftPlan {
     int n = 0, sign = -1, m = 0;
     bool blue = false;
-    // prime-factor + Rader plan (n = S * Q): rad[0, nrad_s) factor S, rad[nrad_s, nrad) factor Q - 1
-    bool rader = false;
+    // prime-factor + Rader plan (n = S * Q): rad[0, nrad_s) factor S, rad[nrad_s, nrad) factor Q - 1;
+    // qdirect: the Q-point transforms run as one direct (symmetric-pair) DFT stage instead
+    // of Rader's convolution, rad[] then factors S only
+    bool rader = false, qdirect = false;
     int S = 1, Q = 1, nrad_s = 0;
     int nrad = 0;
     int rad[24] = {};
     DevBuf chirp;   // n entries exp(sign pi i k^2 / n)        (Bluestein)
     DevBuf bhat;    // m entries FFT_m(conj chirp, wrapped) / m (Bluestein);
                     // Rader: Q - 1 entries FFT_{Q-1}(b) / (Q - 1), b_j = exp(sign 2 pi i g^j / Q)
-    DevBuf pos;     // Rader: 2n shorts, the LDS slot of input element k, then of output element k
+    DevBuf pos;     // Rader / qdirect: 2n shorts, the LDS slot of input element k, then of output element k
     // per stage of the transform's stage list: twiddles exp(sign 2 pi i t k / (Ns R)),
     // t = 1..R-1, for k < Ns, stored [k][t - 1] from float2 offset tw_off[stage]
     DevBuf tw;

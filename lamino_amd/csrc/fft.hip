@@ -75,7 +75,7 @@ struct KParams {
     int S, Q;
     const short *pos_in, *pos_out;
     SubLines whole, sub_s, sub_q;
-    FastDiv fQ1;                          // Q - 1
+    FastDiv fQ1, fS;                      // Q - 1, S
 };
 
 // LDS position of element n of tile line l: n-major when the tile runs along
@@ -274,6 +274,122 @@ __device__ __forceinline__ void stage(float2 *__restrict__ buf, const KParams &P
     __syncthreads();
 }
 
+// Q-point DFTs of a prime-factor plan's rows, directly: with s_j = x_j + x_{Q-j},
+// d_j = x_j - x_{Q-j} (j = 1..H, H = (Q - 1) / 2),
+//   X_k     = A_k + SIGN i B_k,   X_{Q-k} = A_k - SIGN i B_k,
+//   A_k = x_0 + sum_j s_j cos(2 pi j k / Q),   B_k = sum_j d_j sin(2 pi j k / Q),
+// and X_0 = x_0 + sum_j s_j.  Wave w computes the pairs k = 1 + w G .. w G + G for
+// up to 64 rows at once (one row per lane), so k and j are compile-time in its
+// unrolled loops and the cosines and sines are instruction literals.  One
+// compute-barrier-write round replaces Rader's two (Q - 1)-point FFTs and the
+// pointwise step between them (five LDS round trips and their barriers).
+template <int Q>
+struct DqTable {
+    float c[Q], s[Q];
+};
+template <int Q>
+constexpr DqTable<Q> dq_table() {
+    DqTable<Q> t{};
+    for (int m = 0; m < Q; ++m) {
+        t.c[m] = (float)tw_re<Q, 1>(m);
+        t.s[m] = (float)tw_im<Q, 1>(m);
+    }
+    return t;
+}
+
+template <int Q, int K0, int G>
+__device__ __forceinline__ void dq_acc(const float2 *__restrict__ buf, int base, int step, float2 (&A)[G],
+                                       float2 (&B)[G], float2 &X0) {
+    constexpr int H = (Q - 1) / 2;
+    constexpr DqTable<Q> T = dq_table<Q>();
+    const float2 x0 = buf[base];
+    X0 = x0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        A[g] = x0;
+        B[g] = make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        const float2 a = buf[base + j * step], b = buf[base + (Q - j) * step];
+        const float2 sj = cadd(a, b), dj = csub(a, b);
+        if (K0 == 1) X0 = cadd(X0, sj);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int k = K0 + g;
+            if (k <= H) {
+                const float c = T.c[(j * k) % Q], sn = T.s[(j * k) % Q];
+                A[g].x += sj.x * c;
+                A[g].y += sj.y * c;
+                B[g].x += dj.x * sn;
+                B[g].y += dj.y * sn;
+            }
+        }
+    }
+}
+
+template <int Q, int K0, int G>
+__device__ __forceinline__ void dq_write(float2 *__restrict__ buf, int base, int step, const float2 (&A)[G],
+                                         const float2 (&B)[G], float2 X0, float sg) {
+    constexpr int H = (Q - 1) / 2;
+    if (K0 == 1) buf[base] = X0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int k = K0 + g;
+        if (k <= H) {
+            buf[base + k * step] = make_float2(A[g].x - sg * B[g].y, A[g].y + sg * B[g].x);
+            buf[base + (Q - k) * step] = make_float2(A[g].x + sg * B[g].y, A[g].y - sg * B[g].x);
+        }
+    }
+}
+
+template <int Q, int INNER, int NT>
+__device__ __forceinline__ void direct_q_stage(float2 *__restrict__ buf, const KParams &P, int sign) {
+    constexpr int H = (Q - 1) / 2;
+    constexpr int NW = NT / 64;
+    static_assert(NW == 4, "direct DFT stage written for four waves");
+    constexpr int G = (H + NW - 1) / NW;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const float sg = (float)sign;
+    const int nv = P.L * P.S;
+    for (int v0 = 0; v0 < nv; v0 += 64) {
+        const int v = v0 + lane;
+        const bool act = v < nv;
+        int base = 0, step = 1;
+        if (act) {
+            // row c of tile line l: element t at slot c Q + t
+            if (INNER) {
+                const int c = (int)P.fL.div((unsigned)v), l = v - c * P.L;
+                base = c * Q * P.L + l;
+                step = P.L;
+            } else {
+                const int l = (int)P.fS.div((unsigned)v), c = v - l * P.S;
+                base = l * P.m + c * Q;
+            }
+        }
+        float2 A[G], B[G], X0;
+        if (act) {
+            switch (w) {
+                case 0: dq_acc<Q, 1, G>(buf, base, step, A, B, X0); break;
+                case 1: dq_acc<Q, 1 + G, G>(buf, base, step, A, B, X0); break;
+                case 2: dq_acc<Q, 1 + 2 * G, G>(buf, base, step, A, B, X0); break;
+                default: dq_acc<Q, 1 + 3 * G, G>(buf, base, step, A, B, X0); break;
+            }
+        }
+        __syncthreads();   // every wave has read these rows before any overwrites them
+        if (act) {
+            switch (w) {
+                case 0: dq_write<Q, 1, G>(buf, base, step, A, B, X0, sg); break;
+                case 1: dq_write<Q, 1 + G, G>(buf, base, step, A, B, X0, sg); break;
+                case 2: dq_write<Q, 1 + 2 * G, G>(buf, base, step, A, B, X0, sg); break;
+                default: dq_write<Q, 1 + 3 * G, G>(buf, base, step, A, B, X0, sg); break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Rader's step between the two convolution FFTs.  Row c of line l holds y0 (the
 // row's DFT input at index 0) at slot 0 and A = FFT_{Q-1}(a) at slots 1..Q-1,
 // a_p = y[g^-p].  Afterwards slot 0 holds X[0] = y0 + sum_p a_p = y0 + A[0] and
@@ -335,9 +451,13 @@ __device__ __forceinline__ bool tile_line(const KParams &P, int l, int o2base, i
     return true;
 }
 
-template <int INNER, int NT, int MAXLEN, int WAVES>
+// DQ != 0: the instantiation for qdirect plans with prime factor Q = DQ (the S-point
+// stages, then direct_q_stage); its own kernel so that the other plans' passes keep
+// their register allocation.
+template <int INNER, int NT, int MAXLEN, int WAVES, int DQ = 0>
 __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
     extern __shared__ float2 buf[];
+    const int kind = DQ ? (int)KIND_RADER : P.kind;
     const int L = P.L, M = P.m, N = P.n;
     const AxisPass &a = P.a;
     const uint3 bk = tile_block();
@@ -361,7 +481,7 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
         base_in[l] = ok ? o1 * a.is1 + (long long)o2 * a.is2 + (long long)i * a.isi : -1;
         base_out[l] = ok ? o1 * a.os1 + (long long)o2 * a.os2 + (long long)i * a.osi : -1;
     }
-    if (P.kind == KIND_RADER)
+    if (kind == KIND_RADER)
         for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) pos_lds[i] = P.pos_in[i];
     __syncthreads();
 
@@ -430,13 +550,13 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
         for (int u = 0; u < kLoadBatch; ++u) {
             if (line[u] >= 0) {
                 float2 v = val[u];
-                if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
-                const int p = P.kind == KIND_RADER ? (int)pos_lds[nidx[u]] : nidx[u];
+                if (kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
+                const int p = kind == KIND_RADER ? (int)pos_lds[nidx[u]] : nidx[u];
                 buf[lidx<INNER>(line[u], p, L, M)] = v;
             }
         }
     }
-    if (P.kind == KIND_BLUESTEIN) {
+    if (kind == KIND_BLUESTEIN) {
         const int pad = M - N;
         for (int e = threadIdx.x; e < L * pad; e += blockDim.x) {
             int l, n;
@@ -449,6 +569,12 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
 
     for (int s = 0; s < P.nst; ++s) {
         const int gs = P.st_g[s];
+        if constexpr (DQ != 0) {
+            if (gs == 3) {
+                direct_q_stage<DQ, INNER, NT>(buf, P, P.st_sign[s]);
+                continue;
+            }
+        }
         const SubLines &g = gs == 2 ? P.sub_q : gs == 1 ? P.sub_s : P.whole;
         const float2 *tw = P.tw + P.st_tw[s];
 #define LAM_STAGE(R)                                                                                           \
@@ -462,7 +588,7 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
         }
 #undef LAM_STAGE
         if (s == P.pw_after) {
-            if (P.kind == KIND_RADER) {
+            if (kind == KIND_RADER) {
                 rader_pointwise<INNER>(buf, P);
             } else {
                 for (int e = threadIdx.x; e < L * M; e += blockDim.x) {
@@ -504,14 +630,14 @@ __global__ __launch_bounds__(NT, WAVES) void k_fft_axis(KParams P) {
         float2 val[kLoadBatch];
 #pragma unroll
         for (int u = 0; u < kLoadBatch; ++u) {
-            const int p = P.kind == KIND_RADER ? (int)pos_lds[N + nidx[u]] : nidx[u];
+            const int p = kind == KIND_RADER ? (int)pos_lds[N + nidx[u]] : nidx[u];
             val[u] = off[u] >= 0 ? buf[lidx<INNER>(line[u], p, L, M)] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < kLoadBatch; ++u) {
             if (off[u] < 0) continue;
             float2 v = val[u];
-            if (P.kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
+            if (kind == KIND_BLUESTEIN) v = cmul(v, P.chirp[nidx[u]]);
             v.x *= a.scale;
             v.y *= a.scale;
             if (a.st_mode == ST_COMPLEX) static_cast<float2 *>(a.out)[off[u]] = v;
@@ -589,6 +715,10 @@ bool rader_split(int n, int &S, int &Q) {
     return false;
 }
 
+// Prime factors whose transforms run as a direct DFT stage (direct_q_stage has an
+// instantiation for each).  Measured against Rader's convolution at 615 = 15 * 41.
+bool direct_q(int q) { return q == 41 || q == 61; }
+
 int primitive_root(int q) {
     for (int g = 2; g < q; ++g) {
         long long x = 1;
@@ -614,6 +744,8 @@ int inverse_mod(int a, int m) {  // m > 1, gcd(a, m) = 1
 //   X[k] = sum_{n1,n2} x[n1,n2] w_S^{n1 k1} w_Q^{n2 k2}.
 // Within a row, column 0 stays at slot 0, input column g^-p goes to slot 1 + p
 // and output column g^q comes from slot 1 + q (g a primitive root mod Q).
+// A qdirect plan keeps the columns in natural order (slot n1 Q + n2 in, k1 Q + k2 out)
+// and has no bhat.
 void build_rader(Ctx &cx, FftPlan &plan) {
     const int n = plan.n, S = plan.S, Q = plan.Q, q1 = Q - 1;
     const int g = primitive_root(Q);
@@ -627,10 +759,18 @@ void build_rader(Ctx &cx, FftPlan &plan) {
     for (int k = 0; k < n; ++k) {
         const int n1 = S > 1 ? (int)((long long)k * u1 % S) : 0;
         const int n2 = (int)((long long)k * u2 % Q);
-        pos[k] = (short)(n1 * Q + (n2 == 0 ? 0 : 1 + (q1 - lg[n2]) % q1));
         const int k1 = k % S, k2 = k % Q;
-        pos[n + k] = (short)(k1 * Q + (k2 == 0 ? 0 : 1 + lg[k2]));
+        if (plan.qdirect) {
+            pos[k] = (short)(n1 * Q + n2);
+            pos[n + k] = (short)(k1 * Q + k2);
+        } else {
+            pos[k] = (short)(n1 * Q + (n2 == 0 ? 0 : 1 + (q1 - lg[n2]) % q1));
+            pos[n + k] = (short)(k1 * Q + (k2 == 0 ? 0 : 1 + lg[k2]));
+        }
     }
+    plan.pos.alloc(sizeof(short) * pos.size());
+    upload_now(cx, plan.pos.ptr, pos.data(), sizeof(short) * pos.size());
+    if (plan.qdirect) return;
     // bhat = FFT_{Q-1}(b) / (Q - 1), b_j = exp(sign 2 pi i g^j / Q), in double
     const long double twopi = 6.283185307179586476925286766559L;
     std::vector<std::complex<double>> b(q1);
@@ -654,8 +794,6 @@ void build_rader(Ctx &cx, FftPlan &plan) {
     }
     plan.bhat.alloc(sizeof(float2) * q1);
     upload_now(cx, plan.bhat.ptr, bh.data(), sizeof(float2) * q1);
-    plan.pos.alloc(sizeof(short) * pos.size());
-    upload_now(cx, plan.pos.ptr, pos.data(), sizeof(short) * pos.size());
 }
 
 struct StageDesc {
@@ -664,8 +802,8 @@ struct StageDesc {
 
 // The plan's flattened stage list: runs of its radices over a (sub-)line set
 // (g: 0 the whole lines, 1 the S-point columns, 2 the (Q - 1)-point rows of a
-// Rader plan); pw_after is the stage after which Bluestein's or Rader's
-// pointwise product runs (-1: none).
+// Rader plan, 3 the direct Q-point DFT of a qdirect plan's rows); pw_after is the
+// stage after which Bluestein's or Rader's pointwise product runs (-1: none).
 std::vector<StageDesc> stage_list(const FftPlan &plan, int &pw_after) {
     std::vector<StageDesc> st;
     pw_after = -1;
@@ -676,7 +814,10 @@ std::vector<StageDesc> stage_list(const FftPlan &plan, int &pw_after) {
             Ns *= plan.rad[s];
         }
     };
-    if (plan.rader) {
+    if (plan.qdirect) {
+        run(0, plan.nrad_s, plan.S, plan.sign, 1);
+        st.push_back({plan.Q, plan.Q, plan.sign, 3, 1});
+    } else if (plan.rader) {
         const int nq = plan.nrad - plan.nrad_s;
         run(0, plan.nrad_s, plan.S, plan.sign, 1);
         run(plan.nrad_s, nq, plan.Q - 1, -1, 2);
@@ -739,12 +880,13 @@ FftPlan &get_fft_plan(Ctx &cx, int n, int sign) {
         for (int i = 0; i < nrad; ++i) plan->rad[i] = rad[i];
     } else if (n <= kMaxLen && rader_split(n, S, Q)) {
         plan->rader = true;
+        plan->qdirect = direct_q(Q) && n <= kLdsComplex;
         plan->m = n;
         plan->S = S;
         plan->Q = Q;
         int rs[24], ns = 0, rq[24], nq = 0;
         if (S > 1) factor_len(S, rs, ns);
-        factor_len(Q - 1, rq, nq);
+        if (!plan->qdirect) factor_len(Q - 1, rq, nq);
         plan->nrad_s = ns;
         plan->nrad = ns + nq;
         for (int i = 0; i < ns; ++i) plan->rad[i] = rs[i];
@@ -865,6 +1007,7 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
         P.pos_in = plan.pos.as<short>();
         P.pos_out = P.pos_in + plan.n;
         P.fQ1 = FastDiv((unsigned)(plan.Q - 1));
+        P.fS = FastDiv((unsigned)plan.S);
         // S-point transforms: Q columns per line, element t of column c at t * Q + c
         P.sub_s.nv = P.L * plan.Q; P.sub_s.C = plan.Q; P.sub_s.off = 0; P.sub_s.coff = 1; P.sub_s.str = plan.Q;
         P.sub_s.fnv = FastDiv((unsigned)P.sub_s.nv); P.sub_s.fC = FastDiv((unsigned)plan.Q);
@@ -876,7 +1019,20 @@ void run_axis(Ctx &cx, const FftPlan &plan, AxisPass a) {
     const size_t lds = (size_t)P.L * plan.m * sizeof(float2) + 2 * sizeof(long long) * (size_t)P.L +
                        (plan.rader ? 2 * sizeof(short) * (size_t)plan.n : 0);
     TimedScope ts(cx, "fft_axis");
-    if (plan.m <= kLdsComplex) {
+    if (plan.qdirect) {
+        LAM_REQUIRE(plan.m <= kLdsComplex && (plan.Q == 41 || plan.Q == 61), LAM_ERR_STATE, "no direct DFT pass for this plan");
+        if (plan.Q == 41) {
+            if (inner)
+                hipLaunchKernelGGL((k_fft_axis<1, kThreads, kLdsComplex, kFftWaves, 41>), grid, dim3(kThreads), lds, cx.stream, P);
+            else
+                hipLaunchKernelGGL((k_fft_axis<0, kThreads, kLdsComplex, kFftWaves, 41>), grid, dim3(kThreads), lds, cx.stream, P);
+        } else {
+            if (inner)
+                hipLaunchKernelGGL((k_fft_axis<1, kThreads, kLdsComplex, kFftWaves, 61>), grid, dim3(kThreads), lds, cx.stream, P);
+            else
+                hipLaunchKernelGGL((k_fft_axis<0, kThreads, kLdsComplex, kFftWaves, 61>), grid, dim3(kThreads), lds, cx.stream, P);
+        }
+    } else if (plan.m <= kLdsComplex) {
         if (inner)
             hipLaunchKernelGGL((k_fft_axis<1, kThreads, kLdsComplex, kFftWaves>), grid, dim3(kThreads), lds, cx.stream, P);
         else
