@@ -17,13 +17,16 @@ FWD_TOL = 1e-4
 
 
 @pytest.mark.parametrize("shape", [(2, 7, 9), (3, 89, 89), (2, 43, 179), (1, 305, 305), (2, 90, 360), (1, 21, 45),
-                                   (1, 8, 10), (2, 1, 16), (1, 615, 17), (2, 61, 122), (1, 3, 1021), (1, 2, 3600)])
+                                   (1, 8, 10), (2, 1, 16), (1, 615, 17), (2, 61, 122), (1, 3, 1021), (1, 2, 3600),
+                                   (1, 41, 82), (2, 123, 1230), (1, 915, 4), (1, 3, 915), (1, 183, 61), (1, 194, 6)])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft2_matches_numpy(ctx, shape, sign):
     """Covers every plan kind of the pass: 2,3,5-smooth lengths in register
     radices up to 12 (9, 10, 16, 45, 90, 360, 3600 = 12*12*5*5); the Good-Thomas
-    split with Rader's algorithm (7, 17, 21 = 3*7, 61, 122 = 2*61, 305 = 5*61,
-    615 = 15*41); Bluestein (43, 89, 179, 1021), both pass orientations."""
+    split with Rader's algorithm over the prime factor (7, 17, 21 = 3*7, 194 = 2*97) or with
+    the direct DFT stage for the factors 41 and 61 (41, 82, 123, 1230 = 30*41,
+    615 = 15*41 and 915 = 15*61 with their radix-15 stage, 61, 122, 183, 305);
+    Bluestein (43, 89, 179, 1021), both pass orientations."""
     import lamino_amd as la
 
     say("fft2", shape, sign)
