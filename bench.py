@@ -29,6 +29,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
+# The batched and served forms run 4 worker streams beside the main stream.  With
+# HIP's default of 4 hardware queues per process two workers can land on one queue,
+# where their work serialises (measured, 16-job mix: passes at 0.65-1.07 x the plain
+# form among passes at 1.35 x); with 8 queues, 12 of 12 passes ran at 1.36-1.37 x.
+# The runtime reads this once, at its initialisation, which happens after this line.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 METRIC = "fwd+adj NUFFT projections/s, CG iters/s vs plain form; p50/p99 recon latency"
 HBM_PEAK_GBS = 8000.0
 KERNEL_SCOPES = ("interp", "spread", "fft_axis", "deapod_pad", "crop_deapod", "bin_points", "spread_rows2d",

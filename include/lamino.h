@@ -247,7 +247,10 @@ lam_status lam_mbir_footprint(const lam_dataset *datasets, int ndatasets, const 
  * slot frees and the HBM budget allows (lam_mbir_footprint per job).  Jobs on
  * different slots overlap on the device; each job keeps its iterate, Bregman
  * auxiliaries and Toeplitz kernels resident until it finishes.  A job's
- * result does not depend on what runs beside it. */
+ * result does not depend on what runs beside it.  Each worker's stream should
+ * have a hardware queue of its own: run a serving process with
+ * GPU_MAX_HW_QUEUES >= slots + 1 (HIP's default, 4, lets two workers share a
+ * queue, and their work then runs one after the other). */
 typedef struct lam_server lam_server;
 
 enum {
