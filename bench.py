@@ -33,8 +33,14 @@ sys.path.insert(0, HERE)
 # HIP's default of 4 hardware queues per process two workers can land on one queue,
 # where their work serialises (measured, 16-job mix: passes at 0.65-1.07 x the plain
 # form among passes at 1.35 x); with 8 queues, 12 of 12 passes ran at 1.36-1.37 x.
-# The runtime reads this once, at its initialisation, which happens after this line.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# The runtime reads this once, at its initialisation, which happens after this line;
+# an environment that sets fewer (HIP's default is 4) is raised to 8.
+try:
+    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _queues = 4
+if _queues < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "fwd+adj NUFFT projections/s, CG iters/s vs plain form; p50/p99 recon latency"
 HBM_PEAK_GBS = 8000.0
@@ -385,6 +391,7 @@ def run_batched(la, synth, njobs, slots):
            "recon_per_hour_plain": round(3600 * len(jobs) / t_plain, 1),
            "recon_per_hour_batched": round(3600 * len(jobs) / t_batch, 1),
            "speedup": round(t_plain / t_batch, 3), "max_rel_diff_vs_plain": max(diffs),
+           "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
            "peak_admitted_bytes": st["peak_in_use"],
            "mix": "16 jobs: volumes {[16,64,64],[32,128,128],[51,256,256]}x3, 1-3 gammas of 90-360 angles, "
                   "tilt 25-40 deg, MBIR1 (20 CG) or MBIR2 (30x3), seeded"}
