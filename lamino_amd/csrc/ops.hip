@@ -4,6 +4,7 @@
 #include <mutex>
 #include <string>
 
+#include "fastdiv.h"
 #include "fft.h"
 #include "ops.h"
 
@@ -111,17 +112,46 @@ __global__ __launch_bounds__(256) void k_max_partial(const float *a, long long n
 }
 
 // ------------------------------------------------------------------ stencils
+// (i, j, k) of a flat index, k fastest: multiply-shift splits when the volume has
+// fewer than 2^31 cells (a 64-bit division costs ~40 instructions), else division.
+struct Split3 {
+    FastDiv f23, f3;
+    int small = 0;
+};
+Split3 make_split3(const long long d[3]) {
+    Split3 s;
+    s.small = d[0] * d[1] * d[2] < (1ll << 31) ? 1 : 0;
+    if (s.small) {
+        s.f23 = FastDiv((unsigned)(d[1] * d[2]));
+        s.f3 = FastDiv((unsigned)d[2]);
+    }
+    return s;
+}
+__device__ __forceinline__ void split3(long long id, const Split3 &sp, int n2, int n3, int &i, int &j, int &k) {
+    if (sp.small) {
+        const unsigned u = (unsigned)id, q = sp.f23.div(u), r = u - q * sp.f23.d, jq = sp.f3.div(r);
+        i = (int)q;
+        j = (int)jq;
+        k = (int)(r - jq * sp.f3.d);
+    } else {
+        k = (int)(id % n3);
+        j = (int)((id / n3) % n2);
+        i = (int)(id / ((long long)n2 * n3));
+    }
+}
+
 __device__ __forceinline__ bool interior(int i, int j, int k, int n1, int n2, int n3) {
     return i >= 1 && i < n1 - 1 && j >= 1 && j < n2 - 1 && k >= 1 && k < n3 - 1;
 }
 
 __global__ __launch_bounds__(256) void k_grad_u(const float *__restrict__ u, float *g0, float *g1, float *g2, int n1,
-                                                int n2, int n3) {
+                                                int n2, int n3, Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s1 = (long long)n2 * n3, s2 = n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         float a = 0.f, b = 0.f, c = 0.f;
         if (interior(i, j, k, n1, n2, n3)) {
             a = (u[id + s1] - u[id - s1]) * 0.5f;
@@ -133,12 +163,14 @@ __global__ __launch_bounds__(256) void k_grad_u(const float *__restrict__ u, flo
 }
 
 __global__ __launch_bounds__(256) void k_divergence(const float *__restrict__ v0, const float *__restrict__ v1,
-                                                    const float *__restrict__ v2, float *out, int n1, int n2, int n3) {
+                                                    const float *__restrict__ v2, float *out, int n1, int n2, int n3,
+                                                    Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s1 = (long long)n2 * n3, s2 = n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         float r = 0.f;
         if (interior(i, j, k, n1, n2, n3)) {
             const float A = v0[id + s1] - v0[id - s1];
@@ -165,22 +197,25 @@ __device__ __forceinline__ float lap_at(const float *u, long long id, int i, int
     return (float)(-(double)A / 2.0 - (double)B / 2.0 - (double)C / 2.0);
 }
 
-__global__ __launch_bounds__(256) void k_laplacian(const float *__restrict__ u, float *out, int n1, int n2, int n3) {
+__global__ __launch_bounds__(256) void k_laplacian(const float *__restrict__ u, float *out, int n1, int n2, int n3,
+                                                   Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s1 = (long long)n2 * n3, s2 = n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         out[id] = interior(i, j, k, n1, n2, n3) ? lap_at(u, id, i, j, k, n1, n2, n3, s1, s2) : 0.f;
     }
 }
 
-__global__ __launch_bounds__(256) void k_add_mu_lap3(CVol3 x, Vol3 y, float mu, int n1, int n2, int n3) {
+__global__ __launch_bounds__(256) void k_add_mu_lap3(CVol3 x, Vol3 y, float mu, int n1, int n2, int n3, Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s1 = (long long)n2 * n3, s2 = n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         if (!interior(i, j, k, n1, n2, n3)) continue;
 #pragma unroll
         for (int c = 0; c < 3; ++c) y.c[c][id] += lap_at(x.c[c], id, i, j, k, n1, n2, n3, s1, s2) * mu;
@@ -207,12 +242,13 @@ __device__ __forceinline__ float dpot(float delta, float sigma_q, float sigma_qp
 }
 
 __global__ __launch_bounds__(256) void k_qggmrf(float *g, const float *__restrict__ x, int n1, int n2, int n3,
-                                                float sigma_q, float sigma_qp, float p) {
+                                                float sigma_q, float sigma_qp, float p, Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s1 = (long long)n2 * n3, s2 = n3;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s1);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         if (!interior(i, j, k, n1, n2, n3)) continue;
         const float u = x[id];
         float sum = 0.f;
@@ -512,27 +548,28 @@ float max_value(Ctx &cx, long long n, const float *a) {
 void grad_u(Ctx &cx, const long long d[3], const float *u, float *g0, float *g1, float *g2) {
     const long long n = d[0] * d[1] * d[2];
     hipLaunchKernelGGL(k_grad_u, dim3(vgrid(n)), dim3(256), 0, cx.stream, u, g0, g1, g2, (int)d[0], (int)d[1],
-                       (int)d[2]);
+                       (int)d[2], make_split3(d));
     check_launch();
 }
 
 void divergence(Ctx &cx, const long long d[3], const float *v0, const float *v1, const float *v2, float *out) {
     const long long n = d[0] * d[1] * d[2];
     hipLaunchKernelGGL(k_divergence, dim3(vgrid(n)), dim3(256), 0, cx.stream, v0, v1, v2, out, (int)d[0], (int)d[1],
-                       (int)d[2]);
+                       (int)d[2], make_split3(d));
     check_launch();
 }
 
 void laplacian(Ctx &cx, const long long d[3], const float *u, float *out) {
     const long long n = d[0] * d[1] * d[2];
-    hipLaunchKernelGGL(k_laplacian, dim3(vgrid(n)), dim3(256), 0, cx.stream, u, out, (int)d[0], (int)d[1], (int)d[2]);
+    hipLaunchKernelGGL(k_laplacian, dim3(vgrid(n)), dim3(256), 0, cx.stream, u, out, (int)d[0], (int)d[1], (int)d[2],
+                       make_split3(d));
     check_launch();
 }
 
 void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu) {
     const long long n = d[0] * d[1] * d[2];
     hipLaunchKernelGGL(k_add_mu_lap3, dim3(vgrid(n)), dim3(256), 0, cx.stream, x, y, mu, (int)d[0], (int)d[1],
-                       (int)d[2]);
+                       (int)d[2], make_split3(d));
     check_launch();
 }
 
@@ -541,7 +578,7 @@ void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma
     const float sq = (float)std::pow((double)sigma, 2.0);
     const float sqp = (float)std::pow((double)sigma, 2.0 - (double)p);
     hipLaunchKernelGGL(k_qggmrf, dim3(vgrid(n)), dim3(256), 0, cx.stream, g, x, (int)d[0], (int)d[1], (int)d[2], sq,
-                       sqp, p);
+                       sqp, p, make_split3(d));
     check_launch();
 }
 
