@@ -1,6 +1,7 @@
 // Fused elementwise, stencil and reduction kernels on [n1][n2][n3] volumes.
 #pragma once
 #include "common.h"
+#include "fastdiv.h"
 
 namespace lam {
 
@@ -63,5 +64,33 @@ void axpby3(Ctx &cx, long long n, float a, CVol3 x, float b, Vol3 y);
 void copy3(Ctx &cx, long long n, CVol3 x, Vol3 y);
 void fill(Ctx &cx, long long n, float *y, float v);
 void scale_to(Ctx &cx, long long n, const float *x, float s, float *y);
+
+// (i, j, k) of a flat index, k fastest: multiply-shift splits when the volume has
+// fewer than 2^31 cells (a 64-bit division costs ~40 instructions), else division.
+struct Split3 {
+    FastDiv f23, f3;
+    int small = 0;
+};
+inline Split3 make_split3(const long long d[3]) {
+    Split3 s;
+    s.small = d[0] * d[1] * d[2] < (1ll << 31) ? 1 : 0;
+    if (s.small) {
+        s.f23 = FastDiv((unsigned)(d[1] * d[2]));
+        s.f3 = FastDiv((unsigned)d[2]);
+    }
+    return s;
+}
+__device__ __forceinline__ void split3(long long id, const Split3 &sp, int n2, int n3, int &i, int &j, int &k) {
+    if (sp.small) {
+        const unsigned u = (unsigned)id, q = sp.f23.div(u), r = u - q * sp.f23.d, jq = sp.f3.div(r);
+        i = (int)q;
+        j = (int)jq;
+        k = (int)(r - jq * sp.f3.d);
+    } else {
+        k = (int)(id % n3);
+        j = (int)((id / n3) % n2);
+        i = (int)(id / ((long long)n2 * n3));
+    }
+}
 
 }  // namespace lam

@@ -4,7 +4,6 @@
 #include <mutex>
 #include <string>
 
-#include "fastdiv.h"
 #include "fft.h"
 #include "ops.h"
 
@@ -112,34 +111,6 @@ __global__ __launch_bounds__(256) void k_max_partial(const float *a, long long n
 }
 
 // ------------------------------------------------------------------ stencils
-// (i, j, k) of a flat index, k fastest: multiply-shift splits when the volume has
-// fewer than 2^31 cells (a 64-bit division costs ~40 instructions), else division.
-struct Split3 {
-    FastDiv f23, f3;
-    int small = 0;
-};
-Split3 make_split3(const long long d[3]) {
-    Split3 s;
-    s.small = d[0] * d[1] * d[2] < (1ll << 31) ? 1 : 0;
-    if (s.small) {
-        s.f23 = FastDiv((unsigned)(d[1] * d[2]));
-        s.f3 = FastDiv((unsigned)d[2]);
-    }
-    return s;
-}
-__device__ __forceinline__ void split3(long long id, const Split3 &sp, int n2, int n3, int &i, int &j, int &k) {
-    if (sp.small) {
-        const unsigned u = (unsigned)id, q = sp.f23.div(u), r = u - q * sp.f23.d, jq = sp.f3.div(r);
-        i = (int)q;
-        j = (int)jq;
-        k = (int)(r - jq * sp.f3.d);
-    } else {
-        k = (int)(id % n3);
-        j = (int)((id / n3) % n2);
-        i = (int)(id / ((long long)n2 * n3));
-    }
-}
-
 __device__ __forceinline__ bool interior(int i, int j, int k, int n1, int n2, int n3) {
     return i >= 1 && i < n1 - 1 && j >= 1 && j < n2 - 1 && k >= 1 && k < n3 - 1;
 }

@@ -49,12 +49,13 @@ __device__ __forceinline__ bool interior(int i, int j, int k, int n1, int n2, in
 
 // rhs_i = yT_i + mu * divergence(d_i - b_i)   (src/bregman.cpp:81-84)
 __global__ __launch_bounds__(256) void k_sb_rhs(CVol3 yT, Vol9 d, Vol9 b, Vol3 rhs, float mu, int n1, int n2,
-                                                int n3) {
+                                                int n3, Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s[3] = {(long long)n2 * n3, n3, 1};
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s[0]);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         const bool in = interior(i, j, k, n1, n2, n3);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -77,12 +78,13 @@ __global__ __launch_bounds__(256) void k_sb_rhs(CVol3 yT, Vol9 d, Vol9 b, Vol3 r
 
 // isotropic shrinkage + Bregman update (src/bregman.cpp:91-120), gradients of x recomputed in place
 __global__ __launch_bounds__(256) void k_sb_shrink(CVol3 x, Vol9 d, Vol9 b, float eps, float thr, int n1, int n2,
-                                                   int n3) {
+                                                   int n3, Split3 sp) {
     const long long total = (long long)n1 * n2 * n3;
     const long long s[3] = {(long long)n2 * n3, n3, 1};
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int k = (int)(id % n3), j = (int)((id / n3) % n2), i = (int)(id / s[0]);
+        int i, j, k;
+        split3(id, sp, n2, n3, i, j, k);
         const bool in = interior(i, j, k, n1, n2, n3);
         float g[9], t[9];
         float sk = eps;
@@ -253,11 +255,11 @@ void split_bregman(Ctx &cx, const long long d[3], const OpFn &A, bool A_has_lap,
     long long it = 0;
     for (; it < outer_max; ++it) {
         hipLaunchKernelGGL(k_sb_rhs, dim3(vgrid(n)), dim3(256), 0, cx.stream, yT, D, B, rhs.v(), (float)mu, (int)d[0],
-                           (int)d[1], (int)d[2]);
+                           (int)d[1], (int)d[2], make_split3(d));
         check_launch();
         cg_solve(cx, d, Ap, rhs.cv(), xold.cv(), x, inner_max, tol, st);
         hipLaunchKernelGGL(k_sb_shrink, dim3(vgrid(n)), dim3(256), 0, cx.stream, cv(x), D, B, eps, thr, (int)d[0],
-                           (int)d[1], (int)d[2]);
+                           (int)d[1], (int)d[2], make_split3(d));
         check_launch();
         RedTerm t[6];
         for (int c = 0; c < 3; ++c) {
@@ -512,12 +514,12 @@ void split_bregman_graph(Ctx &cx, const long long d[3], const OpFn &A, CVol3 yT,
     // one outer iteration of src/bregman.cpp:75-131
     auto outer_step = [&] {
         hipLaunchKernelGGL(k_sb_rhs, dim3(vgrid(n)), dim3(256), 0, cx.stream, yT, D, B, rhs.v(), (float)mu, (int)d[0],
-                           (int)d[1], (int)d[2]);
+                           (int)d[1], (int)d[2], make_split3(d));
         check_launch();
         fast_cg_begin(w, A, rhs.cv(), xold.cv(), x);
         for (long long k = 0; k < inner_max; ++k) fast_cg_iter(w, A, x, tol);
         hipLaunchKernelGGL(k_sb_shrink, dim3(vgrid(n)), dim3(256), 0, cx.stream, cv(x), D, B, eps, thr, (int)d[0],
-                           (int)d[1], (int)d[2]);
+                           (int)d[1], (int)d[2], make_split3(d));
         check_launch();
         reduce_terms_dev(cx, n, t, 6, xs.as<double>(), w.red.as<double>());
         copy3(cx, n, cv(x), xold.v());
