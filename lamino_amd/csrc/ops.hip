@@ -212,28 +212,101 @@ __device__ __forceinline__ float dpot(float delta, float sigma_q, float sigma_qp
     return (float)(Q / ((double)sigma_q * 0.001));
 }
 
-__global__ __launch_bounds__(256) void k_qggmrf(float *g, const float *__restrict__ x, int n1, int n2, int n3,
-                                                float sigma_q, float sigma_qp, float p, Split3 sp) {
-    const long long total = (long long)n1 * n2 * n3;
-    const long long s1 = (long long)n2 * n3, s2 = n3;
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
-         id += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        split3(id, sp, n2, n3, i, j, k);
-        if (!interior(i, j, k, n1, n2, n3)) continue;
-        const float u = x[id];
-        float sum = 0.f;
+// The prior's 26 terms, each edge evaluated once. For δ ≠ 0, dpot is odd bit for bit:
+// |δ| feeds the power, the sign enters only through δ / (t3·σ_q), and negating a float
+// or double quotient or product is exact; so with v = u + o the term of the edge seen
+// from v is the negated term seen from u (x_v − x_u = −(x_u − x_v) exactly). For δ = 0
+// (and NaN) dpot is the constant 2/(σ_q·0.001) from both ends. A workgroup takes a
+// (j, k) tile of kQJ−2 × kQK−2 interior voxels with a one-voxel halo and marches it
+// along i over kQPlanes planes. On each plane every voxel of the extended tile
+// evaluates its 13 forward edges (o > 0 in (a, b, c) order: 9 to plane i+1, 4 in the
+// plane), keeps them for its own sum and puts the value its neighbour needs (negated
+// unless δ = 0) in LDS; each interior voxel then adds its 26 terms in
+// src/qggmrf.cpp:74-90's (a, b, c) order, the 13 backward ones from LDS (plane i−1's
+// edges to plane i, the plane's own in-plane edges). The sums are those of a per-voxel
+// loop over the 26 neighbours, bit for bit, with 13/26 of the FP64 powers plus the
+// halo's (0.61 of them for a full tile).
+constexpr int kQJ = 16, kQK = 32, kQPlanes = 16;
+
+__global__ __launch_bounds__(kQJ * kQK) void k_qggmrf(float *g, const float *__restrict__ x, int n1, int n2, int n3,
+                                                    float sigma_q, float sigma_qp, float p, int tiles_k) {
+    // Backward values: plane i−1's edges (1, b, c), slot (b+1)·3 + c+1, and plane i's
+    // edges (0,0,1), (0,1,−1), (0,1,0), (0,1,1), each as seen from the edge's far end.
+    __shared__ float up[9][kQJ * kQK];
+    __shared__ float inpl[4][kQJ * kQK];
+    const int t = threadIdx.x, jj = t / kQK, kk = t % kQK;
+    const int j = (blockIdx.x / tiles_k) * (kQJ - 2) + jj;  // the halo row/column is jj, kk = 0
+    const int k = (blockIdx.x % tiles_k) * (kQK - 2) + kk;
+    const int i_lo = 1 + blockIdx.y * kQPlanes, i_hi = min(i_lo + kQPlanes, n1 - 1);
+    const bool inb = j < n2 && k < n3;
+    const bool own = jj >= 1 && jj < kQJ - 1 && kk >= 1 && kk < kQK - 1 && j < n2 - 1 && k < n3 - 1;
+    const long long s1 = (long long)n2 * n3;
+    // Edge values of voxel (i, j, k); an edge leaving the volume is 0 (no interior voxel
+    // reads it). Bit s of `flat` is set where δ is 0 or NaN.
+    auto edge = [&](float d, unsigned &flat, int s) {
+        if (!(fabsf(d) > 0.f)) flat |= 1u << s;
+        return dpot(d, sigma_q, sigma_qp, p);
+    };
+    auto back = [](float e, unsigned flat, int s) { return (flat >> s) & 1u ? e : -e; };
+    auto up_edges = [&](int i, float e[9], unsigned &flat) {
+        const long long id = i * s1 + (long long)j * n3 + k;
+        const float u = inb ? x[id] : 0.f;
 #pragma unroll
-        for (int a = -1; a < 2; ++a)
+        for (int b = -1; b < 2; ++b)
+#pragma unroll
+            for (int c = -1; c < 2; ++c) {
+                float v = 0.f;
+                if (inb && j + b >= 0 && j + b < n2 && k + c >= 0 && k + c < n3)
+                    v = edge(x[id + s1 + b * n3 + c] - u, flat, (b + 1) * 3 + (c + 1));
+                e[(b + 1) * 3 + (c + 1)] = v;
+            }
+    };
+    {
+        float e[9];
+        unsigned flat = 0;
+        up_edges(i_lo - 1, e, flat);
+#pragma unroll
+        for (int s = 0; s < 9; ++s) up[s][t] = back(e[s], flat, s);
+    }
+    for (int i = i_lo; i < i_hi; ++i) {
+        float eu[9], ep[4] = {0.f, 0.f, 0.f, 0.f};
+        unsigned flat = 0;  // bits 0-8: eu, 9-12: ep
+        up_edges(i, eu, flat);
+        const long long id = i * s1 + (long long)j * n3 + k;
+        if (inb) {
+            const float u = x[id];
+            if (k + 1 < n3) ep[0] = edge(x[id + 1] - u, flat, 9);
+            if (j + 1 < n2) {
+#pragma unroll
+                for (int c = -1; c < 2; ++c)
+                    if (k + c >= 0 && k + c < n3) ep[c + 2] = edge(x[id + n3 + c] - u, flat, 11 + c);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) inpl[s][t] = back(ep[s], flat, 9 + s);
+        __syncthreads();
+        if (own) {
+            float sum = 0.f;
 #pragma unroll
             for (int b = -1; b < 2; ++b)
 #pragma unroll
-                for (int c = -1; c < 2; ++c) {
-                    if (a == 0 && b == 0 && c == 0) continue;
-                    const float v = x[id + a * s1 + b * s2 + c];
-                    sum += c_qw[(a + 1) * 9 + (b + 1) * 3 + (c + 1)] * dpot(v - u, sigma_q, sigma_qp, p);
-                }
-        g[id] += sum;
+                for (int c = -1; c < 2; ++c)
+                    sum += c_qw[(b + 1) * 3 + (c + 1)] * up[(1 - b) * 3 + (1 - c)][t + b * kQK + c];
+            sum += c_qw[9] * inpl[3][t - kQK - 1];
+            sum += c_qw[10] * inpl[2][t - kQK];
+            sum += c_qw[11] * inpl[1][t - kQK + 1];
+            sum += c_qw[12] * inpl[0][t - 1];
+            sum += c_qw[14] * ep[0];
+            sum += c_qw[15] * ep[1];
+            sum += c_qw[16] * ep[2];
+            sum += c_qw[17] * ep[3];
+#pragma unroll
+            for (int s = 0; s < 9; ++s) sum += c_qw[18 + s] * eu[s];
+            g[id] += sum;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 9; ++s) up[s][t] = back(eu[s], flat, s);
     }
 }
 
@@ -548,8 +621,13 @@ void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma
     const long long n = d[0] * d[1] * d[2];
     const float sq = (float)std::pow((double)sigma, 2.0);
     const float sqp = (float)std::pow((double)sigma, 2.0 - (double)p);
-    hipLaunchKernelGGL(k_qggmrf, dim3(vgrid(n)), dim3(256), 0, cx.stream, g, x, (int)d[0], (int)d[1], (int)d[2], sq,
-                       sqp, p, make_split3(d));
+    if (d[0] < 3 || d[1] < 3 || d[2] < 3 || n == 0) return;  // no interior voxel
+    const long long tiles_j = (d[1] - 2 + kQJ - 3) / (kQJ - 2), tiles_k = (d[2] - 2 + kQK - 3) / (kQK - 2);
+    const long long chunks = (d[0] - 2 + kQPlanes - 1) / kQPlanes;
+    if (tiles_j * tiles_k > 0x7fffffffLL || chunks > 65535)
+        throw Error(LAM_ERR_ARG, "qggmrf: volume too large for the tiled prior kernel");
+    hipLaunchKernelGGL(k_qggmrf, dim3((unsigned)(tiles_j * tiles_k), (unsigned)chunks), dim3(kQJ * kQK), 0, cx.stream,
+                       g, x, (int)d[0], (int)d[1], (int)d[2], sq, sqp, p, (int)tiles_k);
     check_launch();
 }
 

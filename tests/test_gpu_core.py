@@ -74,13 +74,18 @@ def test_stencils_match_oracle(ctx, orc, dims):
     np.testing.assert_array_equal(la.laplacian(u), lap_ref)
 
 
-@pytest.mark.parametrize("dims", [(6, 5, 7), (9, 12, 11)])
+@pytest.mark.parametrize("dims", [(6, 5, 7), (9, 12, 11), (35, 33, 70)])
 def test_qggmrf_matches_oracle(ctx, orc, dims):
+    """The device kernel evaluates each neighbour pair once and hands the far end the
+    negated value (the δ = 0 constant unnegated); (35, 33, 70) spans several of its
+    14 x 30 (j, k) tiles and 16-plane chunks, with a flat block across their edges."""
     import lamino_amd as la
 
     rng = np.random.default_rng(5)
     x = rng.standard_normal(dims).astype(np.float32)
     x[2, 2, 2] = x[2, 2, 3]  # delta = 0 branch
+    if min(dims) > 20:
+        x[10:20, 10:20, 25:40] = 0.5
     g = rng.standard_normal(dims).astype(np.float32)
     want = orc.qggmrf(g, x, 1000.0, 1.2, prec=32)
     got = la.qggmrf(g.copy(), x, 1000.0, 1.2)
