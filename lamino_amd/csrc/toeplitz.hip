@@ -169,11 +169,20 @@ __global__ __launch_bounds__(256) void k_toep_quad(const float2 *__restrict__ za
     double acc[kMaxQuadGrids * 3];
 #pragma unroll
     for (int q = 0; q < kMaxQuadGrids * 3; ++q) acc[q] = 0.0;
+    // The spectra of real volumes are Hermitian (X(-k) = conj X(k)) and K is real and
+    // even, so the term at -k equals the term at k: the sum runs over one of each pair
+    // of mirrored points with weight 2, and once over the points that are their own
+    // mirror (rows with r = mirror(r) at x = 0 and x = L3/2), which halves the reads.
     const int rows = L1 * L2;
     for (int r = blockIdx.x; r < rows; r += gridDim.x) {
       const SpecRow sr = spec_row(r, L1, L2, L3);
-      for (int x = threadIdx.x; x < L3; x += blockDim.x) {
+      const long long mr = sr.mbase / L3;
+      if (mr < r) continue;  // the mirror row carries these terms
+      const bool self = mr == r;
+      const int xend = self ? L3 / 2 + 1 : L3;
+      for (int x = threadIdx.x; x < xend; x += blockDim.x) {
         const long long id = sr.base + x;
+        const double w = (self && (x == 0 || 2 * x == L3)) ? 1.0 : 2.0;
         float2 X0, X1;
         unpack(za[id], za[sr.mbase + (x ? L3 - x : 0)], X0, X1);
         const float2 X2 = zb[id];
@@ -187,9 +196,9 @@ __global__ __launch_bounds__(256) void k_toep_quad(const float2 *__restrict__ za
                 const double y0x = k0.x * x0x + k0.y * x1x + k1.x * x2x, y0y = k0.x * x0y + k0.y * x1y + k1.x * x2y;
                 const double y1x = k0.y * x0x + k1.y * x1x + k2.x * x2x, y1y = k0.y * x0y + k1.y * x1y + k2.x * x2y;
                 const double y2x = k1.x * x0x + k2.x * x1x + k2.y * x2x, y2y = k1.x * x0y + k2.x * x1y + k2.y * x2y;
-                acc[3 * g + 0] += x0x * y0x + x0y * y0y;
-                acc[3 * g + 1] += x1x * y1x + x1y * y1y;
-                acc[3 * g + 2] += x2x * y2x + x2y * y2y;
+                acc[3 * g + 0] += w * (x0x * y0x + x0y * y0y);
+                acc[3 * g + 1] += w * (x1x * y1x + x1y * y1y);
+                acc[3 * g + 2] += w * (x2x * y2x + x2y * y2y);
             }
         }
       }

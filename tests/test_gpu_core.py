@@ -193,10 +193,14 @@ def test_row_factorised_projectors(ctx, orc, monkeypatch, tilt, slab):
     assert rel(a, a3) < 3e-5
 
 
-def test_sysmat_and_toeplitz_vs_oracle(ctx, orc):
+@pytest.mark.parametrize("dims", [(9, 5, 11), (8, 6, 10)])
+def test_sysmat_and_toeplitz_vs_oracle(ctx, orc, dims):
+    """The quadratic form sums one of each mirrored pair of spectral points twice and the
+    self-mirrored points once; the two shapes give spectral grids (18, 9, 21) and (15, 12,
+    20), odd and even lengths on each axis."""
     import lamino_amd as la
 
-    dims, R, C, P = (9, 5, 11), 13, 15, 10
+    R, C, P = 13, 15, 10
     al1, ga1 = np.full(P, 0.5), np.linspace(0, 2 * np.pi, P, endpoint=False)
     al2, ga2 = np.linspace(-1, 1, P), np.full(P, 0.8)
     g1 = la.PolarGrid.per_projection(al1, ga1, R, C)
