@@ -167,7 +167,7 @@ void mbir_run(Ctx &cx, const lam_dataset *ds, int nds, const long long rd[3], co
             toep_apply(*toep, u, g, 0.f);
             // sum_g (sysmat_g(u) - yT) = N u - G yT   (src/mbir3.cpp:91-97)
             axpby3(cx, n, -G, yT.cv(), 1.f, g);
-            for (int c = 0; c < 3; ++c) qggmrf(cx, od, g.c[c], u.c[c], sigma, p);
+            qggmrf3(cx, od, g, u, 3, sigma, p);
         };
         double yTy = 0;
         {

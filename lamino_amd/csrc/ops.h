@@ -52,6 +52,8 @@ void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu)
 
 // ---- qGGMRF prior gradient (src/qggmrf.cpp:59-95): g_c += prior(x_c), c < ncomp
 void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma, float p);
+// the same for components 0..ncomp-1 of x into those of g, in one launch
+void qggmrf3(Ctx &cx, const long long d[3], Vol3 g, CVol3 x, int ncomp, float sigma, float p);
 
 // ---- ramp preconditioner (include/precond.h:33-87), out = M in, on 1 or 2 volumes at once.
 // work (>= n complex) replaces the context's shared workspace, for work captured into graphs.

@@ -228,8 +228,12 @@ __device__ __forceinline__ float dpot(float delta, float sigma_q, float sigma_qp
 // halo's (0.61 of them for a full tile).
 constexpr int kQJ = 16, kQK = 32, kQPlanes = 16;
 
-__global__ __launch_bounds__(kQJ * kQK) void k_qggmrf(float *g, const float *__restrict__ x, int n1, int n2, int n3,
-                                                    float sigma_q, float sigma_qp, float p, int tiles_k) {
+__global__ __launch_bounds__(kQJ * kQK) void k_qggmrf(Vol3 gv, CVol3 xv, int n1, int n2, int n3, float sigma_q,
+                                                    float sigma_qp, float p, int tiles_k) {
+    // one component per blockIdx.z: the three share one launch (no half-empty last round per component)
+    const unsigned comp = blockIdx.z;
+    float *g = comp == 0 ? gv.c[0] : comp == 1 ? gv.c[1] : gv.c[2];
+    const float *__restrict__ x = comp == 0 ? xv.c[0] : comp == 1 ? xv.c[1] : xv.c[2];
     // Backward values: plane i−1's edges (1, b, c), slot (b+1)·3 + c+1, and plane i's
     // edges (0,0,1), (0,1,−1), (0,1,0), (0,1,1), each as seen from the edge's far end.
     __shared__ float up[9][kQJ * kQK];
@@ -618,6 +622,10 @@ void add_mu_laplacian3(Ctx &cx, const long long d[3], CVol3 x, Vol3 y, float mu)
 }
 
 void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma, float p) {
+    qggmrf3(cx, d, Vol3{{g, nullptr, nullptr}}, CVol3{{x, nullptr, nullptr}}, 1, sigma, p);
+}
+
+void qggmrf3(Ctx &cx, const long long d[3], Vol3 g, CVol3 x, int ncomp, float sigma, float p) {
     const long long n = d[0] * d[1] * d[2];
     const float sq = (float)std::pow((double)sigma, 2.0);
     const float sqp = (float)std::pow((double)sigma, 2.0 - (double)p);
@@ -626,8 +634,11 @@ void qggmrf(Ctx &cx, const long long d[3], float *g, const float *x, float sigma
     const long long chunks = (d[0] - 2 + kQPlanes - 1) / kQPlanes;
     if (tiles_j * tiles_k > 0x7fffffffLL || chunks > 65535)
         throw Error(LAM_ERR_ARG, "qggmrf: volume too large for the tiled prior kernel");
-    hipLaunchKernelGGL(k_qggmrf, dim3((unsigned)(tiles_j * tiles_k), (unsigned)chunks), dim3(kQJ * kQK), 0, cx.stream,
-                       g, x, (int)d[0], (int)d[1], (int)d[2], sq, sqp, p, (int)tiles_k);
+    if (ncomp < 1 || ncomp > 3) throw Error(LAM_ERR_ARG, "qggmrf: 1 to 3 components");
+    for (int c = 0; c < ncomp; ++c)
+        if (!g.c[c] || !x.c[c]) throw Error(LAM_ERR_ARG, "qggmrf: null component");
+    hipLaunchKernelGGL(k_qggmrf, dim3((unsigned)(tiles_j * tiles_k), (unsigned)chunks, (unsigned)ncomp),
+                       dim3(kQJ * kQK), 0, cx.stream, g, x, (int)d[0], (int)d[1], (int)d[2], sq, sqp, p, (int)tiles_k);
     check_launch();
 }
 
